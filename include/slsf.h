@@ -1,0 +1,276 @@
+/*
+ * slsf.h -- C ABI of the MI355X (gfx950) Monte Carlo backend for the 3D
+ * layered-disorder XY model of SLSF-rs.
+ *
+ * The reference binds its hot path in two places, and this ABI replaces both:
+ *   1. the carlo-mc plugin `impl MonteCarlo for ThetaModel`
+ *      (src/model.rs:308-477: new / init / sweep / measure / finalize_estimates /
+ *      task_metadata / build_tasks), and
+ *   2. the free functions it calls (src/updates.rs:440,551;
+ *      src/observables.rs:80,166; src/simulation.rs:13).
+ * The lattice, its trig cache and its RNG state live on the device inside an
+ * opaque batch handle; a single task is a batch of one.  Only FP64 scalars and
+ * bins cross back to the host.  All entry points return 0 on success and a
+ * nonzero SLSF_E_* code otherwise; slsf_last_error() then returns the message,
+ * which for every validation the reference performs is the reference's exact
+ * string (e.g. "temperature T must be positive and finite", src/types.rs:375).
+ *
+ * Threading: a batch handle is used by one host thread at a time; each handle
+ * owns one HIP stream.  Error messages are thread-local.
+ */
+#ifndef SLSF_H
+#define SLSF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLSF_OK 0
+#define SLSF_E_INVALID 1   /* bad argument; message is the reference's string */
+#define SLSF_E_DEVICE 2    /* HIP runtime error, or no usable GPU */
+#define SLSF_E_ADMISSION 3 /* does not fit the configured device-memory budget */
+#define SLSF_E_STATE 4     /* call not valid in the current state */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* slsf_last_error(void);
+/* ABI version, bumped on any incompatible change. */
+int32_t slsf_abi_version(void);
+
+/* ======================================================================== */
+/* Replicas and batches                                                      */
+/* ======================================================================== */
+
+/* Arithmetic of a batch.
+ * SLSF_FP64_REPLAY: FP64 angles and trig cache in HBM, keyed uniforms, the
+ *   library's own FP64 sincos/exp; reproduces the oracle's accept/reject
+ *   decisions and Wolff clusters bit-for-bit.
+ * SLSF_FP32: FP32 sweeps (hardware sin/cos/exp2), FP64 accumulators.  Cubic
+ *   replicas with L in {8,12,16,24,32} run in the register/LDS-resident batched
+ *   kernel, larger ones in the HBM kernels. */
+#define SLSF_FP64_REPLAY 0
+#define SLSF_FP32 1
+
+/* One replica: the GPU counterpart of ThetaLattice + Parameters + proposal
+ * width + wolff_steps + task seed (src/types.rs:84-161; src/model.rs:20-47).
+ * j_xy[z] couples the in-plane bonds of layer z, j_z[z] the bond z -> z+1
+ * (src/updates.rs:131-167).  `seed` keys every uniform of the replica. */
+typedef struct {
+  int32_t l_x, l_y, l_z;
+  int32_t wolff_steps;
+  double temperature;
+  double proposal_width;
+  const double* j_xy; /* [l_z] */
+  const double* j_z;  /* [l_z] */
+  uint64_t seed;
+} slsf_replica_desc;
+
+typedef struct slsf_batch slsf_batch;
+
+/* Replaces ThetaModel::new (src/model.rs:314-371) for n replicas at once.
+ * Validates like the reference: "L_x must be positive" (and y/z), "temperature
+ * T must be positive and finite", "proposal_width must be nonnegative and
+ * finite", "theta simulation requires nonnegative layer couplings", and for the
+ * checkerboard "red-black Metropolis sweep requires even lattice dimensions". */
+int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t precision,
+                      slsf_batch** out);
+void slsf_batch_destroy(slsf_batch* batch);
+int32_t slsf_batch_size(const slsf_batch* batch);
+int32_t slsf_batch_precision(const slsf_batch* batch);
+/* Device bytes held by the handle. */
+int64_t slsf_batch_device_bytes(const slsf_batch* batch);
+
+/* Replaces ThetaModel::init / initialize_angles (src/model.rs:373-377;
+ * src/initialization.rs:48-62).  Random: theta = 2 pi u, u keyed by (seed, site);
+ * cold: theta = 0.  Resets the sweep and cluster counters to 0. */
+int slsf_init_angles(slsf_batch* batch, int32_t cold);
+
+/* Host <-> device copies of one replica's angles (FP64 on the host side), for
+ * checkpoints and tests.  set_theta refreshes the trig cache. */
+int slsf_get_theta(slsf_batch* batch, int32_t replica, double* theta_out);
+int slsf_set_theta(slsf_batch* batch, int32_t replica, const double* theta);
+
+/* Counters of keyed uniforms: Metropolis sweeps and Wolff clusters done so far.
+ * Together with theta they are the whole Markov-chain state. */
+int slsf_get_counters(const slsf_batch* batch, uint32_t* metro_sweeps, uint32_t* wolff_steps);
+int slsf_set_counters(slsf_batch* batch, uint32_t metro_sweeps, uint32_t wolff_steps);
+
+/* Replaces metropolis_sweep_with_scratch (src/updates.rs:440-486): one red/black
+ * sweep of every replica.  acceptance[n] (may be NULL) receives accepted/V;
+ * decisions (may be NULL) receives one byte per site of every replica
+ * (concatenated in replica order) -- 1 where the proposal was accepted. */
+int slsf_metropolis_sweep(slsf_batch* batch, double* acceptance, uint8_t* decisions);
+
+/* Replaces wolff_cluster_step_with_theta_scratch (src/updates.rs:551-620): one
+ * Wolff cluster per replica.  cluster_size[n] may be NULL; members (may be NULL)
+ * receives one byte per site, 1 for the sites that were flipped. */
+int slsf_wolff_cluster_step(slsf_batch* batch, int64_t* cluster_size, uint8_t* members);
+
+/* Replaces ThetaModel::sweep (src/model.rs:379-400): `nsweeps` times one
+ * Metropolis sweep + wolff_steps clusters.  acceptance_sum[n] (may be NULL)
+ * accumulates the per-sweep acceptance like ThetaModel::acceptance_sum. */
+int slsf_sweep(slsf_batch* batch, int32_t nsweeps, double* acceptance_sum);
+
+/* ThetaObservables (src/types.rs:253-263) plus m^4 = (m^2)^2. */
+typedef struct {
+  double energy;                /* E / V */
+  double magnetization_squared; /* |sum e^{i theta}|^2 / V^2 */
+  double cos_x, cos_y, cos_z;   /* sum J cos(theta_i - theta_{i+a}) */
+  double sin_x, sin_y, sin_z;   /* sum J sin(theta_i - theta_{i+a}) */
+} slsf_observables;
+
+/* Replaces measure_theta_observables_with_scratch (src/observables.rs:80-154). */
+int slsf_measure_observables(slsf_batch* batch, slsf_observables* out /* [n] */);
+
+/* Effective correlation radii for a replica (src/observables.rs:177-181);
+ * rmax arguments < 0 mean "None". */
+int slsf_correlation_rmax(const slsf_batch* batch, int32_t replica, int32_t rmax,
+                          int32_t rmax_xy, int32_t rmax_z, int32_t* rxy, int32_t* rz);
+
+/* Replaces measure_theta_correlations_with_scratch (src/observables.rs:166-265)
+ * for one replica.  Buffers: corr_x, corr_y, corr_xy [rxy]; corr_xy_by_z
+ * [l_z * rxy] indexed [z][r-1]; corr_z [rz]. */
+int slsf_measure_correlations(slsf_batch* batch, int32_t replica, int32_t rmax,
+                              int32_t rmax_xy, int32_t rmax_z, double* corr_x,
+                              double* corr_y, double* corr_xy, double* corr_xy_by_z,
+                              double* corr_z);
+
+/* ======================================================================== */
+/* Production runs with on-device binning                                    */
+/* ======================================================================== */
+
+/* Per-measurement series, in the reference's names (src/model.rs:402-442,
+ * 847-855) plus the m^4 extension used for the Binder ratio. */
+#define SLSF_SERIES_ENERGY 0          /* "Energy" */
+#define SLSF_SERIES_ENERGY2 1         /* "EnergySquared" */
+#define SLSF_SERIES_M2 2              /* "MagnetizationSquared" */
+#define SLSF_SERIES_M4 3              /* "MagnetizationFourth" (extension) */
+#define SLSF_SERIES_COS_X 4           /* "_helicity_cos_x" */
+#define SLSF_SERIES_COS_Y 5
+#define SLSF_SERIES_COS_Z 6
+#define SLSF_SERIES_SIN_X 7           /* "_helicity_sin_x" */
+#define SLSF_SERIES_SIN_Y 8
+#define SLSF_SERIES_SIN_Z 9
+#define SLSF_SERIES_SIN2_X 10         /* "_helicity_sin2_x" */
+#define SLSF_SERIES_SIN2_Y 11
+#define SLSF_SERIES_SIN2_Z 12
+#define SLSF_NUM_SERIES 13
+
+/* Name of a series as the reference's results use it. */
+const char* slsf_series_name(int32_t series);
+
+typedef struct {
+  int64_t thermalization;   /* sweeps before measuring (src/model.rs:29) */
+  int64_t sweeps;           /* measured sweeps, one measurement each */
+  int32_t binsize;          /* measurements per bin (carlo-mc binsize) */
+  int32_t sweeps_per_launch;/* 0 = autotuned/default */
+  int32_t use_graph;        /* capture launches in a hipGraph and replay */
+  int32_t reserved;
+} slsf_run_options;
+
+typedef struct {
+  double seconds;           /* wall time of the run (device-synchronised) */
+  double kernel_ms;         /* sum of HIP-event kernel times */
+  int64_t site_updates;     /* sum over replicas of V * sweeps */
+  int64_t launches;         /* kernel launches issued */
+  double acceptance_mean;   /* mean Metropolis acceptance over replicas */
+  double mean_cluster_size; /* mean Wolff cluster size over replicas */
+} slsf_run_stats;
+
+/* Runs thermalization + sweeps for every replica, accumulating FP64 bin means
+ * of every series on the device.  May be called repeatedly; bins append. */
+int slsf_run(slsf_batch* batch, const slsf_run_options* opt, slsf_run_stats* stats);
+
+/* Completed bins of one series of one replica: writes min(cap, nbins) values
+ * and returns the count in *nbins. */
+int slsf_get_bins(slsf_batch* batch, int32_t replica, int32_t series, double* out,
+                  int32_t cap, int32_t* nbins);
+
+/* Mean Metropolis acceptance per replica since init (task_metadata,
+ * src/model.rs:460-467). */
+int slsf_get_acceptance(slsf_batch* batch, double* acceptance /* [n] */);
+
+/* ======================================================================== */
+/* Estimators (host, FP64)                                                   */
+/* ======================================================================== */
+
+/* BinnedEstimate::from_samples as the reference test pins it
+ * (src/tests.rs:436-446): bins are means of `binsize` consecutive samples,
+ * mean = mean of bins, stderr = sample std of bins / sqrt(nbins).  Error:
+ * "binsize is larger than the sample series". */
+int slsf_binned_estimate(const double* samples, int64_t n, int32_t binsize, double* mean,
+                         double* stderr_out, double* bins /* [n/binsize] */,
+                         int64_t* nbins);
+
+/* blocking_stderr (src/simulation.rs:210-226). */
+double slsf_blocking_stderr(const double* x, int64_t n);
+
+/* Derived observables of the reference's Evaluator (src/model.rs:857-888) plus
+ * the Binder ratio, each with a jackknife error over bins.  bins is laid out
+ * [SLSF_NUM_SERIES][nbins]. */
+#define SLSF_EVAL_MAGNETIZATION 0  /* sqrt(<m2>) */
+#define SLSF_EVAL_CHI 1            /* beta V <m2> */
+#define SLSF_EVAL_SPECIFIC_HEAT 2  /* beta^2 V (<E2> - <E>^2) */
+#define SLSF_EVAL_RHO_XY 3         /* (rho_x + rho_y) / 2 */
+#define SLSF_EVAL_RHO_Z 4
+#define SLSF_EVAL_BINDER 5         /* <m4> / <m2>^2 (extension) */
+#define SLSF_EVAL_ENERGY 6         /* <E> */
+#define SLSF_NUM_EVALS 7
+const char* slsf_eval_name(int32_t eval);
+int slsf_evaluate(const double* bins, int64_t nbins, double volume, double beta,
+                  double* value /* [SLSF_NUM_EVALS] */, double* error /* [SLSF_NUM_EVALS] */);
+
+/* Jackknife of f(mean of columns) for a user function given as values of the
+ * leave-one-out estimates: loo[nbins], full estimate f_all.  Returns the
+ * bias-corrected estimate and the jackknife standard error. */
+int slsf_jackknife(const double* loo, int64_t nbins, double f_all, double* estimate,
+                   double* stderr_out);
+
+/* ======================================================================== */
+/* Job configuration (TOML) and task expansion                               */
+/* ======================================================================== */
+
+/* A task as ThetaTask (src/model.rs:20-47).  Couplings are per layer. */
+typedef struct {
+  char name[128];
+  int32_t l, l_x, l_y, l_z;
+  double temperature, j_xy, delta_j_xy, j_z_mean, delta_j_z;
+  uint64_t disorder_seed, seed;
+  int32_t sample;
+  int64_t sweeps, thermalization;
+  int32_t binsize;
+  double proposal_width;
+  int32_t wolff_steps;
+  int32_t correlation_rmax, correlation_rmax_xy, correlation_rmax_z;
+  int32_t correlation_interval;
+  int32_t disorder_kind; /* 0 two-point (reference), 1 uniform (extension) */
+} slsf_task;
+
+typedef struct slsf_job slsf_job;
+
+/* Parses a job TOML with the reference's schema and aliases
+ * (src/model.rs:159-213,549-644) and expands it into tasks with the
+ * reference's names, seeds and disorder draws (src/model.rs:697-810).
+ * Extension keys: [model] disorder = "two_point" | "uniform",
+ * j_xy_min/j_xy_max (uniform bounds).  Errors carry the reference's messages. */
+int slsf_job_from_toml(const char* toml_text, slsf_job** out);
+void slsf_job_destroy(slsf_job* job);
+const char* slsf_job_name(const slsf_job* job);
+int64_t slsf_job_num_tasks(const slsf_job* job);
+int slsf_job_task(const slsf_job* job, int64_t index, slsf_task* out);
+/* Layer couplings of a task: j_xy[l_z], j_z[l_z]. */
+int slsf_job_task_couplings(const slsf_job* job, int64_t index, double* j_xy, double* j_z);
+/* Durations run_time / checkpoint_time in seconds (src/model.rs:1033-1060). */
+int slsf_job_durations(const slsf_job* job, int64_t* run_time_s, int64_t* checkpoint_time_s);
+/* Round-robin selection idx % world == rank (src/model.rs:62-69). */
+int64_t slsf_job_selected_tasks(const slsf_job* job, int32_t rank, int32_t world,
+                                int64_t* indices, int64_t cap);
+/* "[D-][[HH:]MM:]SS" -> seconds (src/model.rs:1033-1060). */
+int slsf_parse_duration(const char* text, int64_t* seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLSF_H */

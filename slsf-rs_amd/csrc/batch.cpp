@@ -1,0 +1,725 @@
+// batch.cpp -- C-ABI entry points that own device state: replica batches, the
+// reference-mirroring update/measure calls, and production runs with on-device
+// binning.  Compiled by hipcc as host code (no kernels here).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/slsf.h"
+#include "common.h"
+#include "launch.h"
+#include "resident.h"
+
+namespace slsf {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+void clear_error() { g_error.clear(); }
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+bool temperature_ok(double t) { return t > 0.0 && std::isfinite(t); }
+bool width_ok(double w) { return w >= 0.0 && std::isfinite(w); }
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(SLSF_E_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                     " at " #expr);                                   \
+  } while (0)
+
+struct HostReplica {
+  int lx, ly, lz, wolff_steps;
+  double temperature, width;
+  std::vector<double> j_xy, j_z;
+  uint64_t seed;
+  int64_t volume, padded, site_off;
+  int jz_off;
+};
+
+}  // namespace slsf
+
+using namespace slsf;
+
+struct slsf_batch {
+  int precision = SLSF_FP64_REPLAY;
+  hipStream_t stream = nullptr;
+  std::vector<HostReplica> reps;
+  std::vector<RepDev> reps_host;  // mirror of the device descriptors
+  GlobalState g;
+  int64_t total_layers = 0;
+  int64_t device_bytes = 0;
+  uint8_t* d_decisions = nullptr;
+  double* d_accum = nullptr;     // [nrep][NUM_SERIES]
+  double* d_bins = nullptr;      // [nrep][NUM_SERIES][bin_cap]
+  int32_t* d_bin_state = nullptr;  // [nrep][2]: measurements in open bin, completed bins
+  int32_t bin_cap = 0;
+  int32_t binsize = 0;
+  std::vector<double> acc_sum;   // per replica acceptance sums
+  std::vector<int64_t> acc_count;
+  std::vector<double> cluster_sum;
+  std::vector<int64_t> cluster_count;
+  std::vector<void*> allocations;
+  ResidentPlan resident;         // FP32 register/LDS-resident groups
+  bool initialised = false;
+
+  ~slsf_batch() {
+    for (void* p : allocations) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  template <typename T>
+  int alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIP_TRY(hipMalloc(&q, bytes));
+    allocations.push_back(q);
+    device_bytes += (int64_t)bytes;
+    *p = (T*)q;
+    return SLSF_OK;
+  }
+};
+
+namespace {
+
+int validate_replica(const slsf_replica_desc& d) {
+  if (d.l_x <= 0) return fail(SLSF_E_INVALID, "L_x must be positive");
+  if (d.l_y <= 0) return fail(SLSF_E_INVALID, "L_y must be positive");
+  if (d.l_z <= 0) return fail(SLSF_E_INVALID, "L_z must be positive");
+  if (!temperature_ok(d.temperature))
+    return fail(SLSF_E_INVALID, "temperature T must be positive and finite");
+  if (!width_ok(d.proposal_width))
+    return fail(SLSF_E_INVALID, "proposal_width must be nonnegative and finite");
+  if (d.wolff_steps < 0) return fail(SLSF_E_INVALID, "wolff_steps must be nonnegative");
+  if (!d.j_xy || !d.j_z) return fail(SLSF_E_INVALID, "layer coupling arrays are required");
+  for (int z = 0; z < d.l_z; ++z)
+    if (!(d.j_xy[z] >= 0.0) || !(d.j_z[z] >= 0.0))
+      return fail(SLSF_E_INVALID, "theta simulation requires nonnegative layer couplings");
+  if (d.l_x % 2 || d.l_y % 2 || d.l_z % 2)
+    return fail(SLSF_E_INVALID, "red-black Metropolis sweep requires even lattice dimensions");
+  const int64_t v = (int64_t)d.l_x * d.l_y * d.l_z;
+  if (v >= (int64_t)1 << 31) return fail(SLSF_E_INVALID, "lattice volume must be below 2^31");
+  return SLSF_OK;
+}
+
+int device_ready() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(SLSF_E_DEVICE, "no usable GPU (hipGetDeviceCount found none)");
+  return SLSF_OK;
+}
+
+template <typename R>
+int upload_couplings(slsf_batch* b) {
+  std::vector<R> jxy(b->total_layers), jz(b->total_layers);
+  for (const auto& r : b->reps)
+    for (int z = 0; z < r.lz; ++z) {
+      jxy[r.jz_off + z] = (R)r.j_xy[z];
+      jz[r.jz_off + z] = (R)r.j_z[z];
+    }
+  HIP_TRY(hipMemcpyAsync(b->g.jxy, jxy.data(), sizeof(R) * jxy.size(), hipMemcpyHostToDevice,
+                         b->stream));
+  HIP_TRY(hipMemcpyAsync(b->g.jz, jz.data(), sizeof(R) * jz.size(), hipMemcpyHostToDevice,
+                         b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return SLSF_OK;
+}
+
+int sync_reps_from_device(slsf_batch* b) {
+  HIP_TRY(hipMemcpyAsync(b->reps_host.data(), b->g.reps, sizeof(RepDev) * b->reps_host.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return SLSF_OK;
+}
+
+int push_reps_to_device(slsf_batch* b) {
+  HIP_TRY(hipMemcpyAsync(b->g.reps, b->reps_host.data(), sizeof(RepDev) * b->reps_host.size(),
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return SLSF_OK;
+}
+
+bool is_fp64(const slsf_batch* b) { return b->precision == SLSF_FP64_REPLAY; }
+
+int metro_sweep(slsf_batch* b, uint8_t* d_decisions) {
+  launch_clear_counters(b->g, b->stream);
+  if (is_fp64(b)) launch_metro_sweep<double>(b->g, d_decisions, b->stream);
+  else launch_metro_sweep<float>(b->g, d_decisions, b->stream);
+  HIP_TRY(hipGetLastError());
+  return SLSF_OK;
+}
+
+int wolff_round(slsf_batch* b, int round, uint8_t* members_host) {
+  if (is_fp64(b)) launch_wolff<double>(b->g, round, b->stream);
+  else launch_wolff<float>(b->g, round, b->stream);
+  HIP_TRY(hipGetLastError());
+  if (members_host) {
+    std::vector<uint32_t> words(b->g.total_sites / 32);
+    HIP_TRY(hipMemcpyAsync(words.data(), b->g.visited, sizeof(uint32_t) * words.size(),
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int64_t out = 0;
+    for (const auto& r : b->reps)
+      for (int64_t i = 0; i < r.volume; ++i, ++out) {
+        const int64_t gi = r.site_off + i;
+        members_host[out] = (uint8_t)((words[gi >> 5] >> (gi & 31)) & 1u);
+      }
+  }
+  if (is_fp64(b)) launch_wolff_flip_clear<double>(b->g, round, b->stream);
+  else launch_wolff_flip_clear<float>(b->g, round, b->stream);
+  HIP_TRY(hipGetLastError());
+  return SLSF_OK;
+}
+
+int max_wolff_steps(const slsf_batch* b) {
+  int m = 0;
+  for (const auto& r : b->reps) m = std::max(m, r.wolff_steps);
+  return m;
+}
+
+int gather_acceptance(slsf_batch* b, double* acceptance) {
+  std::vector<unsigned long long> acc(b->reps.size());
+  HIP_TRY(hipMemcpyAsync(acc.data(), b->g.acc, sizeof(unsigned long long) * acc.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t r = 0; r < acc.size(); ++r) {
+    const double a = (double)acc[r] / (double)b->reps[r].volume;
+    b->acc_sum[r] += a;
+    b->acc_count[r] += 1;
+    if (acceptance) acceptance[r] += a;
+  }
+  return SLSF_OK;
+}
+
+int gather_clusters(slsf_batch* b, int round, int64_t* cluster_size) {
+  std::vector<long long> cs(b->reps.size());
+  HIP_TRY(hipMemcpyAsync(cs.data(), b->g.csize, sizeof(long long) * cs.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t r = 0; r < cs.size(); ++r) {
+    if (round < b->reps[r].wolff_steps) {
+      b->cluster_sum[r] += (double)cs[r];
+      b->cluster_count[r] += 1;
+    }
+    if (cluster_size) cluster_size[r] = cs[r];
+  }
+  return SLSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* slsf_last_error(void) { return g_error.c_str(); }
+int32_t slsf_abi_version(void) { return 1; }
+
+int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t precision,
+                      slsf_batch** out) {
+  clear_error();
+  if (!out) return fail(SLSF_E_INVALID, "output handle pointer is null");
+  *out = nullptr;
+  if (n <= 0 || !replicas) return fail(SLSF_E_INVALID, "a batch needs at least one replica");
+  if (precision != SLSF_FP64_REPLAY && precision != SLSF_FP32)
+    return fail(SLSF_E_INVALID, "precision must be SLSF_FP64_REPLAY or SLSF_FP32");
+  for (int i = 0; i < n; ++i)
+    if (int rc = validate_replica(replicas[i])) return rc;
+  if (int rc = device_ready()) return rc;
+
+  auto b = std::make_unique<slsf_batch>();
+  b->precision = precision;
+  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  int64_t site_off = 0;
+  int jz_off = 0;
+  for (int i = 0; i < n; ++i) {
+    const auto& d = replicas[i];
+    HostReplica r;
+    r.lx = d.l_x; r.ly = d.l_y; r.lz = d.l_z;
+    r.wolff_steps = d.wolff_steps;
+    r.temperature = d.temperature;
+    r.width = d.proposal_width;
+    r.j_xy.assign(d.j_xy, d.j_xy + d.l_z);
+    r.j_z.assign(d.j_z, d.j_z + d.l_z);
+    r.seed = d.seed;
+    r.volume = (int64_t)d.l_x * d.l_y * d.l_z;
+    r.padded = (r.volume + 63) / 64 * 64;
+    r.site_off = site_off;
+    r.jz_off = jz_off;
+    site_off += r.padded;
+    jz_off += d.l_z;
+    b->reps.push_back(std::move(r));
+  }
+  b->total_layers = jz_off;
+  GlobalState& g = b->g;
+  g.nrep = n;
+  g.total_sites = site_off;
+  for (const auto& r : b->reps) g.max_volume = std::max(g.max_volume, r.volume);
+  g.obs_blocks = (int)((g.max_volume + 255) / 256);
+  const size_t rs = precision == SLSF_FP64_REPLAY ? sizeof(double) : sizeof(float);
+  slsf_batch* bp = b.get();
+  int rc = SLSF_OK;
+  rc = rc ? rc : bp->alloc((RepDev**)&g.reps, (size_t)n);
+  rc = rc ? rc : bp->alloc((char**)&g.theta, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.cth, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.sth, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.jxy, rs * (size_t)jz_off);
+  rc = rc ? rc : bp->alloc((char**)&g.jz, rs * (size_t)jz_off);
+  rc = rc ? rc : bp->alloc(&g.bondmask, (size_t)site_off);
+  rc = rc ? rc : bp->alloc(&g.visited, (size_t)site_off / 32);
+  rc = rc ? rc : bp->alloc(&g.qa, (size_t)site_off);
+  rc = rc ? rc : bp->alloc(&g.qb, (size_t)site_off);
+  rc = rc ? rc : bp->alloc(&g.wd, (size_t)n);
+  rc = rc ? rc : bp->alloc(&g.csize, (size_t)n);
+  rc = rc ? rc : bp->alloc(&g.acc, (size_t)n);
+  rc = rc ? rc : bp->alloc(&g.partials, (size_t)n * g.obs_blocks * 9);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(g.visited, 0, sizeof(uint32_t) * (size_t)(site_off / 32), b->stream));
+  HIP_TRY(hipMemsetAsync(g.theta, 0, rs * (size_t)site_off, b->stream));
+  b->reps_host.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const auto& r = b->reps[i];
+    RepDev& d = b->reps_host[i];
+    std::memset(&d, 0, sizeof d);
+    d.lx = r.lx; d.ly = r.ly; d.lz = r.lz; d.wolff_steps = r.wolff_steps;
+    d.site_off = r.site_off;
+    d.volume = r.volume;
+    d.jz_off = r.jz_off;
+    d.key0 = (uint32_t)r.seed;
+    d.key1 = (uint32_t)(r.seed >> 32);
+    d.beta = 1.0 / r.temperature;  // src/updates.rs:453
+    d.width = r.width;
+  }
+  if ((rc = push_reps_to_device(bp))) return rc;
+  rc = precision == SLSF_FP64_REPLAY ? upload_couplings<double>(bp) : upload_couplings<float>(bp);
+  if (rc) return rc;
+  b->acc_sum.assign(n, 0.0);
+  b->acc_count.assign(n, 0);
+  b->cluster_sum.assign(n, 0.0);
+  b->cluster_count.assign(n, 0);
+  if (precision == SLSF_FP32) {
+    if ((rc = resident_plan(bp->reps_host, bp->resident))) return rc;
+  }
+  *out = b.release();
+  return SLSF_OK;
+}
+
+void slsf_batch_destroy(slsf_batch* batch) { delete batch; }
+int32_t slsf_batch_size(const slsf_batch* b) { return b ? (int32_t)b->reps.size() : 0; }
+int32_t slsf_batch_precision(const slsf_batch* b) { return b ? b->precision : -1; }
+int64_t slsf_batch_device_bytes(const slsf_batch* b) { return b ? b->device_bytes : 0; }
+
+int slsf_init_angles(slsf_batch* b, int32_t cold) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  for (auto& d : b->reps_host) { d.metro_ctr = 0; d.wolff_ctr = 0; }
+  if (int rc = push_reps_to_device(b)) return rc;
+  if (is_fp64(b)) launch_init<double>(b->g, cold, b->stream);
+  else launch_init<float>(b->g, cold, b->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  std::fill(b->acc_sum.begin(), b->acc_sum.end(), 0.0);
+  std::fill(b->acc_count.begin(), b->acc_count.end(), 0);
+  std::fill(b->cluster_sum.begin(), b->cluster_sum.end(), 0.0);
+  std::fill(b->cluster_count.begin(), b->cluster_count.end(), 0);
+  b->initialised = true;
+  return SLSF_OK;
+}
+
+int slsf_get_theta(slsf_batch* b, int32_t replica, double* theta_out) {
+  clear_error();
+  if (!b || !theta_out) return fail(SLSF_E_INVALID, "null argument");
+  if (replica < 0 || replica >= (int)b->reps.size())
+    return fail(SLSF_E_INVALID, "replica index out of range");
+  const auto& r = b->reps[replica];
+  if (is_fp64(b)) {
+    HIP_TRY(hipMemcpyAsync(theta_out, (double*)b->g.theta + r.site_off,
+                           sizeof(double) * r.volume, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+  } else {
+    std::vector<float> tmp(r.volume);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)b->g.theta + r.site_off,
+                           sizeof(float) * r.volume, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (int64_t i = 0; i < r.volume; ++i) theta_out[i] = tmp[i];
+  }
+  return SLSF_OK;
+}
+
+int slsf_set_theta(slsf_batch* b, int32_t replica, const double* theta) {
+  clear_error();
+  if (!b || !theta) return fail(SLSF_E_INVALID, "null argument");
+  if (replica < 0 || replica >= (int)b->reps.size())
+    return fail(SLSF_E_INVALID, "replica index out of range");
+  const auto& r = b->reps[replica];
+  if (is_fp64(b)) {
+    HIP_TRY(hipMemcpyAsync((double*)b->g.theta + r.site_off, theta, sizeof(double) * r.volume,
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    launch_refresh<double>(b->g, b->stream);
+  } else {
+    std::vector<float> tmp(theta, theta + r.volume);
+    HIP_TRY(hipMemcpyAsync((float*)b->g.theta + r.site_off, tmp.data(), sizeof(float) * r.volume,
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    launch_refresh<float>(b->g, b->stream);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->initialised = true;
+  return SLSF_OK;
+}
+
+int slsf_get_counters(const slsf_batch* bc, uint32_t* metro, uint32_t* wolff) {
+  clear_error();
+  slsf_batch* b = const_cast<slsf_batch*>(bc);
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (int rc = sync_reps_from_device(b)) return rc;
+  for (size_t r = 0; r < b->reps.size(); ++r) {
+    if (metro) metro[r] = b->reps_host[r].metro_ctr;
+    if (wolff) wolff[r] = b->reps_host[r].wolff_ctr;
+  }
+  return SLSF_OK;
+}
+
+int slsf_set_counters(slsf_batch* b, uint32_t metro, uint32_t wolff) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (int rc = sync_reps_from_device(b)) return rc;
+  for (auto& d : b->reps_host) { d.metro_ctr = metro; d.wolff_ctr = wolff; }
+  return push_reps_to_device(b);
+}
+
+int slsf_metropolis_sweep(slsf_batch* b, double* acceptance, uint8_t* decisions) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (decisions && !b->d_decisions)
+    if (int rc = b->alloc(&b->d_decisions, (size_t)b->g.total_sites)) return rc;
+  if (acceptance) std::fill(acceptance, acceptance + b->reps.size(), 0.0);
+  if (int rc = metro_sweep(b, decisions ? b->d_decisions : nullptr)) return rc;
+  if (int rc = gather_acceptance(b, acceptance)) return rc;
+  if (decisions) {
+    std::vector<uint8_t> all(b->g.total_sites);
+    HIP_TRY(hipMemcpyAsync(all.data(), b->d_decisions, all.size(), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int64_t o = 0;
+    for (const auto& r : b->reps) {
+      std::memcpy(decisions + o, all.data() + r.site_off, (size_t)r.volume);
+      o += r.volume;
+    }
+  }
+  return SLSF_OK;
+}
+
+int slsf_wolff_cluster_step(slsf_batch* b, int64_t* cluster_size, uint8_t* members) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  // a standalone cluster step runs one cluster for every replica (round 0 with
+  // wolff_steps forced >= 1 would change the descriptor; use a dedicated path)
+  if (int rc = sync_reps_from_device(b)) return rc;
+  std::vector<int> saved(b->reps_host.size());
+  bool patched = false;
+  for (size_t r = 0; r < saved.size(); ++r) {
+    saved[r] = b->reps_host[r].wolff_steps;
+    if (saved[r] < 1) { b->reps_host[r].wolff_steps = 1; patched = true; }
+  }
+  if (patched) if (int rc = push_reps_to_device(b)) return rc;
+  int rc = wolff_round(b, 0, members);
+  if (!rc) rc = gather_clusters(b, 0, cluster_size);
+  if (patched) {
+    if (int rc2 = sync_reps_from_device(b)) return rc2;
+    for (size_t r = 0; r < saved.size(); ++r) b->reps_host[r].wolff_steps = saved[r];
+    if (int rc2 = push_reps_to_device(b)) return rc2;
+  }
+  return rc;
+}
+
+int slsf_sweep(slsf_batch* b, int32_t nsweeps, double* acceptance_sum) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (nsweeps < 0) return fail(SLSF_E_INVALID, "nsweeps must be nonnegative");
+  const int wmax = max_wolff_steps(b);
+  for (int s = 0; s < nsweeps; ++s) {
+    if (int rc = metro_sweep(b, nullptr)) return rc;
+    if (int rc = gather_acceptance(b, acceptance_sum)) return rc;
+    for (int k = 0; k < wmax; ++k) {
+      if (int rc = wolff_round(b, k, nullptr)) return rc;
+      if (int rc = gather_clusters(b, k, nullptr)) return rc;
+    }
+  }
+  return SLSF_OK;
+}
+
+int slsf_measure_observables(slsf_batch* b, slsf_observables* out) {
+  clear_error();
+  if (!b || !out) return fail(SLSF_E_INVALID, "null argument");
+  if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
+  else launch_observables<float>(b->g, b->stream);
+  HIP_TRY(hipGetLastError());
+  const int nb = b->g.obs_blocks;
+  std::vector<double> part((size_t)b->reps.size() * nb * 9);
+  HIP_TRY(hipMemcpyAsync(part.data(), b->g.partials, sizeof(double) * part.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t r = 0; r < b->reps.size(); ++r) {
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int blk = 0; blk < nb; ++blk)
+      for (int k = 0; k < 9; ++k) v[k] += part[((size_t)r * nb + blk) * 9 + k];
+    const double vol = (double)b->reps[r].volume;
+    out[r].energy = v[0] / vol;
+    out[r].magnetization_squared = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
+    out[r].cos_x = v[3]; out[r].cos_y = v[4]; out[r].cos_z = v[5];
+    out[r].sin_x = v[6]; out[r].sin_y = v[7]; out[r].sin_z = v[8];
+  }
+  return SLSF_OK;
+}
+
+int slsf_correlation_rmax(const slsf_batch* b, int32_t replica, int32_t rmax, int32_t rmax_xy,
+                          int32_t rmax_z, int32_t* rxy, int32_t* rz) {
+  clear_error();
+  if (!b || !rxy || !rz) return fail(SLSF_E_INVALID, "null argument");
+  if (replica < 0 || replica >= (int)b->reps.size())
+    return fail(SLSF_E_INVALID, "replica index out of range");
+  const auto& r = b->reps[replica];
+  // src/observables.rs:177-181
+  const int req_xy = rmax_xy >= 0 ? rmax_xy : (rmax >= 0 ? rmax : std::min(r.lx / 2, r.ly / 2));
+  const int req_z = rmax_z >= 0 ? rmax_z : (rmax >= 0 ? rmax : r.lz / 2);
+  *rxy = std::min(std::min(req_xy, r.lx / 2), r.ly / 2);
+  *rz = std::min(req_z, r.lz / 2);
+  return SLSF_OK;
+}
+
+int slsf_measure_correlations(slsf_batch* b, int32_t replica, int32_t rmax, int32_t rmax_xy,
+                              int32_t rmax_z, double* corr_x, double* corr_y, double* corr_xy,
+                              double* corr_xy_by_z, double* corr_z) {
+  int32_t rxy = 0, rz = 0;
+  if (int rc = slsf_correlation_rmax(b, replica, rmax, rmax_xy, rmax_z, &rxy, &rz)) return rc;
+  const auto& r = b->reps[replica];
+  const int64_t rows = (int64_t)r.ly * r.lz;
+  double* d_xy = nullptr;
+  double* d_z = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&d_xy, sizeof(double) * (size_t)std::max<int64_t>(1, rxy * rows * 3), b->stream));
+  HIP_TRY(hipMallocAsync((void**)&d_z, sizeof(double) * (size_t)std::max<int64_t>(1, rz * rows), b->stream));
+  if (is_fp64(b)) launch_corr<double>(b->g, replica, rxy, rz, d_xy, d_z, b->stream);
+  else launch_corr<float>(b->g, replica, rxy, rz, d_xy, d_z, b->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> hxy((size_t)rxy * rows * 3), hz((size_t)rz * rows);
+  if (!hxy.empty())
+    HIP_TRY(hipMemcpyAsync(hxy.data(), d_xy, sizeof(double) * hxy.size(), hipMemcpyDeviceToHost,
+                           b->stream));
+  if (!hz.empty())
+    HIP_TRY(hipMemcpyAsync(hz.data(), d_z, sizeof(double) * hz.size(), hipMemcpyDeviceToHost,
+                           b->stream));
+  HIP_TRY(hipFreeAsync(d_xy, b->stream));
+  HIP_TRY(hipFreeAsync(d_z, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  // host reduction in the reference's order (src/observables.rs:189-252)
+  const double vol = (double)r.volume, area = (double)r.lx * r.ly;
+  for (int rr = 1; rr <= rxy; ++rr) {
+    double sx = 0, sy = 0;
+    for (int z = 0; z < r.lz; ++z) {
+      double lsx = 0, lsy = 0;
+      for (int y = 0; y < r.ly; ++y) {
+        const double* o = &hxy[(((size_t)(rr - 1) * rows) + (size_t)z * r.ly + y) * 3];
+        lsx += o[0];
+        lsx += o[1];
+        lsy += o[2];
+      }
+      sx += lsx;
+      sy += lsy;
+      if (corr_xy_by_z) corr_xy_by_z[(size_t)z * rxy + (rr - 1)] = (lsx + lsy) / (2.0 * area);
+    }
+    if (corr_x) corr_x[rr - 1] = sx / vol;
+    if (corr_y) corr_y[rr - 1] = sy / vol;
+    if (corr_xy) corr_xy[rr - 1] = (sx / vol + sy / vol) / 2.0;
+  }
+  for (int rr = 1; rr <= rz; ++rr) {
+    double sz = 0;
+    for (int z = 0; z < r.lz; ++z)
+      for (int y = 0; y < r.ly; ++y) sz += hz[((size_t)(rr - 1) * rows) + (size_t)z * r.ly + y];
+    if (corr_z) corr_z[rr - 1] = sz / vol;
+  }
+  return SLSF_OK;
+}
+
+int slsf_get_acceptance(slsf_batch* b, double* acceptance) {
+  clear_error();
+  if (!b || !acceptance) return fail(SLSF_E_INVALID, "null argument");
+  for (size_t r = 0; r < b->reps.size(); ++r)
+    acceptance[r] = b->acc_count[r] ? b->acc_sum[r] / (double)b->acc_count[r] : 0.0;
+  return SLSF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// production runs
+// ---------------------------------------------------------------------------
+namespace slsf {
+void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
+                       int32_t binsize, int32_t bin_cap, hipStream_t s);
+}
+
+namespace {
+
+int ensure_bins(slsf_batch* b, const slsf_run_options* opt) {
+  const int64_t need_bins = opt->binsize > 0 ? opt->sweeps / opt->binsize : 0;
+  if (b->d_bins && b->binsize != opt->binsize)
+    return fail(SLSF_E_STATE, "binsize must not change between runs of one batch");
+  int32_t have = 0;
+  if (b->d_bins) {
+    // grow the bin store keeping completed bins
+    std::vector<int32_t> st(b->reps.size() * 2);
+    HIP_TRY(hipMemcpy(st.data(), b->d_bin_state, sizeof(int32_t) * st.size(),
+                      hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < b->reps.size(); ++r) have = std::max(have, st[2 * r + 1]);
+    if (have + need_bins <= b->bin_cap) return SLSF_OK;
+    const int32_t cap = (int32_t)(have + need_bins + 1);
+    double* nb = nullptr;
+    if (int rc = b->alloc(&nb, b->reps.size() * SLSF_NUM_SERIES * (size_t)cap)) return rc;
+    for (size_t k = 0; k < b->reps.size() * SLSF_NUM_SERIES; ++k)
+      HIP_TRY(hipMemcpy(nb + k * cap, b->d_bins + k * b->bin_cap, sizeof(double) * b->bin_cap,
+                        hipMemcpyDeviceToDevice));
+    b->d_bins = nb;
+    b->bin_cap = cap;
+    return SLSF_OK;
+  }
+  b->binsize = opt->binsize;
+  b->bin_cap = (int32_t)std::max<int64_t>(1, need_bins + 1);
+  if (int rc = b->alloc(&b->d_bins, b->reps.size() * SLSF_NUM_SERIES * (size_t)b->bin_cap)) return rc;
+  if (int rc = b->alloc(&b->d_accum, b->reps.size() * SLSF_NUM_SERIES)) return rc;
+  if (int rc = b->alloc(&b->d_bin_state, b->reps.size() * 2)) return rc;
+  HIP_TRY(hipMemset(b->d_accum, 0, sizeof(double) * b->reps.size() * SLSF_NUM_SERIES));
+  HIP_TRY(hipMemset(b->d_bin_state, 0, sizeof(int32_t) * b->reps.size() * 2));
+  return SLSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* slsf_series_name(int32_t s) {
+  static const char* names[SLSF_NUM_SERIES] = {
+      "Energy",          "EnergySquared",   "MagnetizationSquared", "MagnetizationFourth",
+      "_helicity_cos_x", "_helicity_cos_y", "_helicity_cos_z",      "_helicity_sin_x",
+      "_helicity_sin_y", "_helicity_sin_z", "_helicity_sin2_x",     "_helicity_sin2_y",
+      "_helicity_sin2_z"};
+  return (s >= 0 && s < SLSF_NUM_SERIES) ? names[s] : "";
+}
+
+int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) {
+  clear_error();
+  if (!b || !opt) return fail(SLSF_E_INVALID, "null argument");
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (opt->thermalization < 0 || opt->sweeps < 0)
+    return fail(SLSF_E_INVALID, "sweep counts must be nonnegative");
+  if (opt->sweeps > 0 && opt->binsize <= 0) return fail(SLSF_E_INVALID, "binsize must be positive");
+  if (opt->sweeps > 0 && opt->sweeps < opt->binsize)
+    return fail(SLSF_E_INVALID, "sweeps must be at least binsize");
+  if (opt->sweeps > 0)
+    if (int rc = ensure_bins(b, opt)) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t launches = 0;
+  double kernel_ms = 0.0;
+  if (b->precision == SLSF_FP32 && b->resident.covers_all) {
+    ResidentRunArgs a;
+    a.thermalization = opt->thermalization;
+    a.sweeps = opt->sweeps;
+    a.binsize = opt->binsize;
+    a.sweeps_per_launch = opt->sweeps_per_launch;
+    a.use_graph = opt->use_graph;
+    a.accum = b->d_accum;
+    a.bins = b->d_bins;
+    a.bin_state = b->d_bin_state;
+    a.bin_cap = b->bin_cap;
+    a.acc_out = b->g.acc;
+    a.csize_out = b->g.csize;
+    ResidentRunResult res;
+    if (int rc = resident_run(b->resident, b->g, a, b->stream, &res)) return rc;
+    launches = res.launches;
+    kernel_ms = res.kernel_ms;
+    std::vector<unsigned long long> acc(b->reps.size());
+    std::vector<long long> cs(b->reps.size());
+    HIP_TRY(hipMemcpyAsync(acc.data(), b->g.acc, sizeof(unsigned long long) * acc.size(),
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpyAsync(cs.data(), b->g.csize, sizeof(long long) * cs.size(),
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const int64_t nsw = opt->thermalization + opt->sweeps;
+    for (size_t r = 0; r < b->reps.size(); ++r) {
+      b->acc_sum[r] += (double)acc[r] / (double)b->reps[r].volume;
+      b->acc_count[r] += nsw;
+      b->cluster_sum[r] += (double)cs[r];
+      b->cluster_count[r] += nsw * b->reps[r].wolff_steps;
+    }
+  } else {
+    const int wmax = max_wolff_steps(b);
+    const int64_t total = opt->thermalization + opt->sweeps;
+    for (int64_t s = 0; s < total; ++s) {
+      if (int rc = metro_sweep(b, nullptr)) return rc;
+      if (int rc = gather_acceptance(b, nullptr)) return rc;
+      for (int k = 0; k < wmax; ++k) {
+        if (int rc = wolff_round(b, k, nullptr)) return rc;
+        if (int rc = gather_clusters(b, k, nullptr)) return rc;
+      }
+      launches += 4 + 5 * wmax;
+      if (s >= opt->thermalization) {
+        if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
+        else launch_observables<float>(b->g, b->stream);
+        launch_accumulate(b->g, b->d_accum, b->d_bins, b->d_bin_state, b->binsize, b->bin_cap,
+                          b->stream);
+        launches += 2;
+        HIP_TRY(hipGetLastError());
+      }
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (stats) {
+    stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    stats->kernel_ms = kernel_ms;
+    int64_t su = 0;
+    double acc = 0, cl = 0;
+    int64_t ncl = 0;
+    for (size_t r = 0; r < b->reps.size(); ++r) {
+      su += b->reps[r].volume * (opt->thermalization + opt->sweeps);
+      acc += b->acc_count[r] ? b->acc_sum[r] / (double)b->acc_count[r] : 0.0;
+      cl += b->cluster_sum[r];
+      ncl += b->cluster_count[r];
+    }
+    stats->site_updates = su;
+    stats->launches = launches;
+    stats->acceptance_mean = acc / (double)b->reps.size();
+    stats->mean_cluster_size = ncl ? cl / (double)ncl : 0.0;
+  }
+  return SLSF_OK;
+}
+
+int slsf_get_bins(slsf_batch* b, int32_t replica, int32_t series, double* out, int32_t cap,
+                  int32_t* nbins) {
+  clear_error();
+  if (!b || !nbins) return fail(SLSF_E_INVALID, "null argument");
+  if (replica < 0 || replica >= (int)b->reps.size())
+    return fail(SLSF_E_INVALID, "replica index out of range");
+  if (series < 0 || series >= SLSF_NUM_SERIES) return fail(SLSF_E_INVALID, "unknown series");
+  if (!b->d_bins) { *nbins = 0; return SLSF_OK; }
+  int32_t st[2];
+  HIP_TRY(hipMemcpy(st, b->d_bin_state + 2 * replica, sizeof st, hipMemcpyDeviceToHost));
+  *nbins = st[1];
+  const int32_t n = std::min(cap, st[1]);
+  if (n > 0 && out)
+    HIP_TRY(hipMemcpy(out, b->d_bins + ((size_t)replica * SLSF_NUM_SERIES + series) * b->bin_cap,
+                      sizeof(double) * n, hipMemcpyDeviceToHost));
+  return SLSF_OK;
+}
+
+}  // extern "C"

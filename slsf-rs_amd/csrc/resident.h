@@ -1,0 +1,46 @@
+// resident.h -- the FP32 batched kernel in which each replica lives in the
+// registers and LDS of one workgroup for a whole launch of many sweeps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "launch.h"
+
+namespace slsf {
+
+struct ResidentGroup {
+  int L = 0;                    // cubic edge of every replica in the group
+  std::vector<int32_t> replicas;  // batch indices
+  int32_t* d_index = nullptr;   // device copy of `replicas`
+};
+
+struct ResidentPlan {
+  bool covers_all = false;      // every replica is cubic with a resident L
+  std::vector<ResidentGroup> groups;
+};
+
+struct ResidentRunArgs {
+  int64_t thermalization = 0, sweeps = 0;
+  int32_t binsize = 1, sweeps_per_launch = 0, use_graph = 0;
+  double* accum = nullptr;
+  double* bins = nullptr;
+  int32_t* bin_state = nullptr;
+  int32_t bin_cap = 0;
+  unsigned long long* acc_out = nullptr;  // accepted proposals per replica (summed)
+  long long* csize_out = nullptr;         // cluster sites per replica (summed)
+};
+
+struct ResidentRunResult {
+  int64_t launches = 0;
+  double kernel_ms = 0.0;
+};
+
+// L values the resident kernel is built for.
+bool resident_supports(int L);
+int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan);
+int resident_run(ResidentPlan& plan, const GlobalState& g, const ResidentRunArgs& a,
+                 hipStream_t s, ResidentRunResult* out);
+
+}  // namespace slsf
