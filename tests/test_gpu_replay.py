@@ -181,17 +181,3 @@ def test_run_bins_match_oracle_series():
     assert np.allclose(got, expect, rtol=1e-11, atol=1e-12)
 
 
-def test_validation_errors_are_reference_strings():
-    ok = dict(l_x=4, l_y=4, l_z=4, temperature=1.0, j_xy=np.ones(4), j_z=np.ones(4), seed=1)
-    cases = [
-        (dict(ok, temperature=0.0), "temperature T must be positive and finite"),
-        (dict(ok, temperature=float("nan")), "temperature T must be positive and finite"),
-        (dict(ok, proposal_width=-0.1), "proposal_width must be nonnegative and finite"),
-        (dict(ok, l_x=3, l_y=3, l_z=3, j_xy=np.ones(3), j_z=np.ones(3)),
-         "red-black Metropolis sweep requires even lattice dimensions"),
-        (dict(ok, j_z=-np.ones(4)), "theta simulation requires nonnegative layer couplings"),
-    ]
-    for kw, msg in cases:
-        with pytest.raises(slsf.SlsfError) as e:
-            slsf.Batch([slsf.Replica(**kw)])
-        assert e.value.message == msg
