@@ -167,8 +167,14 @@ typedef struct {
   int32_t binsize;          /* measurements per bin (carlo-mc binsize) */
   int32_t sweeps_per_launch;/* 0 = autotuned/default */
   int32_t use_graph;        /* capture launches in a hipGraph and replay */
-  int32_t reserved;
+  int32_t stream;           /* FP32 resident kernel: SLSF_STREAM_PRODUCTION (one
+                               keyed Philox call per two site updates) or
+                               SLSF_STREAM_REPLAY (the FP64 replay path's per-site
+                               uniforms, for decision-mismatch statistics) */
 } slsf_run_options;
+
+#define SLSF_STREAM_PRODUCTION 0
+#define SLSF_STREAM_REPLAY 1
 
 typedef struct {
   double seconds;           /* wall time of the run (device-synchronised) */

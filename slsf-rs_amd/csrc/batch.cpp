@@ -572,34 +572,36 @@ void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_
 namespace {
 
 int ensure_bins(slsf_batch* b, const slsf_run_options* opt) {
-  const int64_t need_bins = opt->binsize > 0 ? opt->sweeps / opt->binsize : 0;
-  if (b->d_bins && b->binsize != opt->binsize)
-    return fail(SLSF_E_STATE, "binsize must not change between runs of one batch");
-  int32_t have = 0;
-  if (b->d_bins) {
-    // grow the bin store keeping completed bins
-    std::vector<int32_t> st(b->reps.size() * 2);
-    HIP_TRY(hipMemcpy(st.data(), b->d_bin_state, sizeof(int32_t) * st.size(),
-                      hipMemcpyDeviceToHost));
-    for (size_t r = 0; r < b->reps.size(); ++r) have = std::max(have, st[2 * r + 1]);
-    if (have + need_bins <= b->bin_cap) return SLSF_OK;
-    const int32_t cap = (int32_t)(have + need_bins + 1);
-    double* nb = nullptr;
-    if (int rc = b->alloc(&nb, b->reps.size() * SLSF_NUM_SERIES * (size_t)cap)) return rc;
-    for (size_t k = 0; k < b->reps.size() * SLSF_NUM_SERIES; ++k)
-      HIP_TRY(hipMemcpy(nb + k * cap, b->d_bins + k * b->bin_cap, sizeof(double) * b->bin_cap,
-                        hipMemcpyDeviceToDevice));
-    b->d_bins = nb;
-    b->bin_cap = cap;
-    return SLSF_OK;
+  const size_t nrep = b->reps.size();
+  if (!b->d_accum) {
+    if (int rc = b->alloc(&b->d_accum, nrep * SLSF_NUM_SERIES)) return rc;
+    if (int rc = b->alloc(&b->d_bin_state, nrep * 2)) return rc;
+    HIP_TRY(hipMemset(b->d_accum, 0, sizeof(double) * nrep * SLSF_NUM_SERIES));
+    HIP_TRY(hipMemset(b->d_bin_state, 0, sizeof(int32_t) * nrep * 2));
   }
+  if (opt->sweeps == 0) return SLSF_OK;
+  std::vector<int32_t> st(nrep * 2);
+  HIP_TRY(hipMemcpy(st.data(), b->d_bin_state, sizeof(int32_t) * st.size(),
+                    hipMemcpyDeviceToHost));
+  int32_t have = 0;
+  bool measured = false;
+  for (size_t r = 0; r < nrep; ++r) {
+    have = std::max(have, st[2 * r + 1]);
+    measured = measured || st[2 * r] != 0 || st[2 * r + 1] != 0;
+  }
+  if (b->binsize != 0 && b->binsize != opt->binsize && measured)
+    return fail(SLSF_E_STATE, "binsize must not change between measured runs of one batch");
   b->binsize = opt->binsize;
-  b->bin_cap = (int32_t)std::max<int64_t>(1, need_bins + 1);
-  if (int rc = b->alloc(&b->d_bins, b->reps.size() * SLSF_NUM_SERIES * (size_t)b->bin_cap)) return rc;
-  if (int rc = b->alloc(&b->d_accum, b->reps.size() * SLSF_NUM_SERIES)) return rc;
-  if (int rc = b->alloc(&b->d_bin_state, b->reps.size() * 2)) return rc;
-  HIP_TRY(hipMemset(b->d_accum, 0, sizeof(double) * b->reps.size() * SLSF_NUM_SERIES));
-  HIP_TRY(hipMemset(b->d_bin_state, 0, sizeof(int32_t) * b->reps.size() * 2));
+  const int64_t need = (int64_t)have + opt->sweeps / opt->binsize + 1;
+  if (need <= b->bin_cap) return SLSF_OK;
+  double* nb = nullptr;
+  if (int rc = b->alloc(&nb, nrep * SLSF_NUM_SERIES * (size_t)need)) return rc;
+  if (b->d_bins)  // keep completed bins
+    for (size_t k = 0; k < nrep * SLSF_NUM_SERIES; ++k)
+      HIP_TRY(hipMemcpy(nb + k * need, b->d_bins + k * b->bin_cap, sizeof(double) * b->bin_cap,
+                        hipMemcpyDeviceToDevice));
+  b->d_bins = nb;
+  b->bin_cap = (int32_t)need;
   return SLSF_OK;
 }
 
@@ -625,8 +627,9 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
   if (opt->sweeps > 0 && opt->binsize <= 0) return fail(SLSF_E_INVALID, "binsize must be positive");
   if (opt->sweeps > 0 && opt->sweeps < opt->binsize)
     return fail(SLSF_E_INVALID, "sweeps must be at least binsize");
-  if (opt->sweeps > 0)
-    if (int rc = ensure_bins(b, opt)) return rc;
+  if (opt->stream != SLSF_STREAM_PRODUCTION && opt->stream != SLSF_STREAM_REPLAY)
+    return fail(SLSF_E_INVALID, "stream must be SLSF_STREAM_PRODUCTION or SLSF_STREAM_REPLAY");
+  if (int rc = ensure_bins(b, opt)) return rc;
   const auto t0 = std::chrono::steady_clock::now();
   int64_t launches = 0;
   double kernel_ms = 0.0;
@@ -637,6 +640,7 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     a.binsize = opt->binsize;
     a.sweeps_per_launch = opt->sweeps_per_launch;
     a.use_graph = opt->use_graph;
+    a.replay_stream = opt->stream == SLSF_STREAM_REPLAY;
     a.accum = b->d_accum;
     a.bins = b->d_bins;
     a.bin_state = b->d_bin_state;

@@ -24,6 +24,7 @@ struct ResidentPlan {
 struct ResidentRunArgs {
   int64_t thermalization = 0, sweeps = 0;
   int32_t binsize = 1, sweeps_per_launch = 0, use_graph = 0;
+  int32_t replay_stream = 0;  // 1: per-site keyed uniforms of the replay path
   double* accum = nullptr;
   double* bins = nullptr;
   int32_t* bin_state = nullptr;

@@ -1,17 +1,530 @@
-// resident.hip -- placeholder until the resident kernel lands: no group is
-// planned, so FP32 batches run on the HBM kernels.
-#include "resident.h"
+// resident.hip -- the FP32 batched kernel: each replica lives in one workgroup
+// for a whole launch of many sweeps (Metropolis + Wolff + measurement).
+//
+// Mapping (cubic L in {8, 12, 16, 24, 32}).  Thread t owns the z-column
+// (x, y) = (t % RW, t / RW), RW = next power of two >= L (lanes x >= L idle).
+// A site's state is the unit vector (cos theta, sin theta) in two FP32
+// registers; a Metropolis proposal theta -> theta + delta is the rotation by
+// delta, so one hardware sincos (of delta) is the only transcendental a site
+// update needs besides the Boltzmann factor.  Couplings J_xy[z], J_z[z],
+// J_z[z-1] sit in an LDS table.
+//
+// Metropolis, colour p (src/updates.rs:440-486): all threads write the
+// (cos, sin) of their opposite-colour sites into LDS slot idx/2 (every pair of
+// x-neighbours holds one site of each colour), barrier, then every thread
+// updates its colour-p sites reading the 4 in-plane neighbours from LDS and the
+// 2 z-neighbours from its own registers.
+//
+// Wolff (src/updates.rs:551-620): r_i = cos(theta_i - phi) for every site goes
+// to LDS, then every site decides its three canonical bonds (+x, +y, +z) with
+// its keyed uniforms; the decisions become LDS bitmaps, one 32-bit word per
+// (y, z) row (wave ballots).  The cluster is the seed's component of that bond
+// graph -- the same set the reference's DFS builds from the same uniforms --
+// found by a monotone flood fill over rows (x-runs closed inside the word, y
+// and z by neighbouring words) until a round changes nothing.  Members are
+// reflected in registers.
+//
+// Measurement (src/observables.rs:80-154): after the colour-1 exchange of the
+// next sweep, each colour-0 site sums its six bonds (each bond has exactly one
+// colour-0 end), all sites sum (cos, sin); FP32 per thread, FP64 across the
+// workgroup, accumulated into FP64 bins on the device.
+//
+// Uniforms are the library's keyed Philox stream (same counters as the FP64
+// replay path; FP32 uses the top 24 bits of the words), so a replica's chain
+// does not depend on the batch it runs in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+
 #include "../../include/slsf.h"
+#include "common.h"
+#include "resident.h"
 
 namespace slsf {
-bool resident_supports(int) { return false; }
-int resident_plan(const std::vector<RepDev>&, ResidentPlan& plan) {
-  plan.covers_all = false;
+
+template <int L>
+struct RShape {
+  static constexpr int RW = L <= 8 ? 8 : (L <= 16 ? 16 : 32);
+  static constexpr int NT = RW * L;
+  static constexpr int V = L * L * L;
+  static constexpr int ROWS = L * L;
+  static constexpr int NW = NT / 64;
+  static constexpr int RPW = 64 / RW;  // y values per wave
+  static constexpr uint32_t MASK = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
+  static_assert(NT % 64 == 0, "workgroup must be whole waves");
+  static_assert((L / 2) % 2 == 0, "pairs of same-colour sites per column");
+};
+
+struct ResArgs {
+  RepDev* reps;
+  const int32_t* index;  // batch replica index per workgroup
+  float* theta;
+  float* cth;
+  float* sth;
+  const float* jxy;
+  const float* jz;
+  double* accum;
+  double* bins;
+  int32_t* bin_state;
+  unsigned long long* acc_out;
+  long long* csize_out;
+  int32_t bin_cap, binsize, nsweeps, measure;
+};
+
+template <int L>
+SLSF_D uint32_t rotl1(uint32_t w) {
+  return ((w << 1) | (w >> (L - 1))) & RShape<L>::MASK;
+}
+template <int L>
+SLSF_D uint32_t rotr1(uint32_t w) {
+  return (w >> 1) | ((w & 1u) << (L - 1));
+}
+
+// An empty asm makes a value opaque to the optimizer.  sel() makes both
+// operands opaque before selecting, so `cond ? a[i] : a[j]` stays a select of
+// two registers instead of a load from a selected address (which would move
+// the register array to scratch), and coordinates made opaque at the start of
+// every phase keep per-site address arithmetic from being hoisted out of the
+// sweep loop into hundreds of live registers.
+SLSF_D float opq(float v) {
+  asm("" : "+v"(v));
+  return v;
+}
+SLSF_D float sel(bool cond, float a, float b) {
+  const float oa = opq(a), ob = opq(b);
+  return cond ? oa : ob;
+}
+
+// hardware transcendentals: v_sin/v_cos take revolutions, v_exp is 2^x
+SLSF_D float sin_t(float t) { return __builtin_amdgcn_sinf(t); }
+SLSF_D float cos_t(float t) { return __builtin_amdgcn_cosf(t); }
+SLSF_D float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+SLSF_D float fract_hw(float x) { return __builtin_amdgcn_fractf(x); }
+
+// sum of N doubles over the workgroup; result valid in thread 0
+template <int N, int NW>
+SLSF_D void wg_sum(double (&v)[N], double* red) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) red[wave * N + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < NW; ++w) s += red[w * N + k];
+      v[k] = s;
+    }
+}
+
+// PAIRED: one Philox call serves two same-colour sites of a column (production
+// stream, tag METRO_PAIR, counter index = first site of the pair); otherwise
+// one call per site with the replay path's counters (tag METRO), which lets an
+// FP32 run consume exactly the uniforms the FP64 replay path uses.
+template <int L, bool PAIRED>
+__global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
+  using S = RShape<L>;
+  __shared__ float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V]
+  __shared__ uint32_t bxm[S::ROWS], bym[S::ROWS], bzm[S::ROWS], vis[S::ROWS];
+  __shared__ float4 jt[L];           // {J_xy[z], J_z[z], J_z[z-1], 0}
+  __shared__ double red[S::NW * 9];
+  __shared__ int flags[3];
+  float* rbuf = reinterpret_cast<float*>(xbuf);
+
+  const int rep = a.index[blockIdx.x];
+  const RepDev d = a.reps[rep];
+  const int t = threadIdx.x;
+  int x = t % S::RW, y = t / S::RW;
+  const bool own = x < L;
+  const int64_t base = d.site_off;
+  const float beta_l2e = (float)(d.beta * 1.4426950408889634);
+  const float wturn = (float)(d.width * 0.15915494309189535);  // width in turns
+  const uint32_t k0 = d.key0, k1 = d.key1;
+  uint32_t mctr = d.metro_ctr, wctr = d.wolff_ctr;
+
+  // angles of this thread's column in turns, [0, 1)
+  float th[L];
+#pragma unroll
+  for (int z = 0; z < L; ++z) {
+    th[z] = 0.0f;
+    if (own) th[z] = fract_hw(a.theta[base + x + L * (y + L * z)] * INV_TWO_PI_F);
+  }
+  if (t < L) {
+    const int zm = (t == 0) ? L - 1 : t - 1;
+    jt[t] = make_float4(a.jxy[d.jz_off + t], a.jz[d.jz_off + t], a.jz[d.jz_off + zm], 0.0f);
+  }
+  if (t < 3) flags[t] = 0;
+  __syncthreads();
+
+  uint32_t nacc_wave = 0;  // wave-uniform: accepted proposals of this wave
+  long long ncluster = 0;  // thread 0: cluster sites
+
+  // write (cos, sin) of this thread's sites of colour `col` to LDS slot idx/2
+  auto exchange = [&](int col) __attribute__((always_inline)) {
+    asm volatile("" : "+v"(x), "+v"(y));
+    const int z1 = (col + x + y) & 1;
+    if (!own) return;
+#pragma unroll
+    for (int k = 0; k < L / 2; ++k) {
+      const int z = 2 * k + z1;
+      const float tv = sel(z1, th[2 * k + 1], th[2 * k]);
+      xbuf[(x + L * (y + L * z)) >> 1] = make_float2(cos_t(tv), sin_t(tv));
+    }
+  };
+
+  // Metropolis update of this thread's colour-p sites
+  auto update = [&](int p) __attribute__((always_inline)) {
+    asm volatile("" : "+v"(x), "+v"(y));
+    const int z0 = (p + x + y) & 1;
+    if (!own) return;
+    const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+    const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+    Words w{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < L / 2; ++k) {
+      const int z = 2 * k + z0;
+      const int ka = 2 * k, kb = 2 * k + 1;
+      const float t0 = sel(z0, th[kb], th[ka]);
+      const int zn = (z + 1 == L) ? 0 : z + 1, zq = (z == 0) ? L - 1 : z - 1;
+      const int row = L * (y + L * z);
+      const float2 nxp = xbuf[(xp + row) >> 1];
+      const float2 nxm = xbuf[(xm + row) >> 1];
+      const float2 nyp = xbuf[(x + L * (yp + L * z)) >> 1];
+      const float2 nym = xbuf[(x + L * (ym + L * z)) >> 1];
+      const float2 nzp = xbuf[(x + L * (y + L * zn)) >> 1];
+      const float2 nzm = xbuf[(x + L * (y + L * zq)) >> 1];
+      const float4 jj = jt[z];
+      const float hx = jj.x * (nxp.x + nxm.x + nyp.x + nym.x) + jj.y * nzp.x + jj.z * nzm.x;
+      const float hy = jj.x * (nxp.y + nxm.y + nyp.y + nym.y) + jj.y * nzp.y + jj.z * nzm.y;
+      float up, ua;
+      if (PAIRED) {
+        if (k % 2 == 0) w = keyed(k0, k1, (uint32_t)(x + row), mctr, TAG_METRO_PAIR);
+        up = u24(k % 2 == 0 ? w.w0 : w.w2);
+        ua = u24(k % 2 == 0 ? w.w1 : w.w3);
+      } else {
+        w = keyed(k0, k1, (uint32_t)(x + row), mctr, TAG_METRO);
+        up = u24(w.w0);
+        ua = u24(w.w2);
+      }
+      const float t1 = fract_hw(t0 + wturn * (2.0f * up - 1.0f));
+      const float c0 = cos_t(t0), s0 = sin_t(t0), c1 = cos_t(t1), s1 = sin_t(t1);
+      const float de = (c0 - c1) * hx + (s0 - s1) * hy;
+      const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
+      const bool acc = (de <= 0.0f) | (ua < pacc);
+      nacc_wave += __popcll(__ballot(acc));  // scalar count per wave
+      const bool wa = acc & !z0, wb = acc & (z0 != 0);
+      th[ka] = wa ? t1 : th[ka];
+      th[kb] = wb ? t1 : th[kb];
+      // one site in flight per thread: keeps the register budget of the
+      // 1024-thread L=32 workgroup (occupancy supplies the latency hiding)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+
+  // observables of the current state; needs the colour-1 exchange in LDS
+  auto measure = [&]() __attribute__((always_inline)) {
+    asm volatile("" : "+v"(x), "+v"(y));
+    float e_c[3] = {0.f, 0.f, 0.f}, e_s[3] = {0.f, 0.f, 0.f};
+    float mx = 0.f, my = 0.f;
+    if (own) {
+      const int z0 = (x + y) & 1;  // colour-0 sites of the column: z = 2k + z0
+      const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+      const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+#pragma unroll
+      for (int k = 0; k < L / 2; ++k) {
+        const int z = 2 * k + z0;
+        const int ka = 2 * k, kb = 2 * k + 1;
+        const int kn = (2 * k + 2) % L, kp = (2 * k - 1 + L) % L;
+        const float t0 = sel(z0, th[kb], th[ka]);
+        const float cc = cos_t(t0), ss = sin_t(t0);
+        const int row = L * (y + L * z);
+        const float2 nxp = xbuf[(xp + row) >> 1];
+        const float2 nxm = xbuf[(xm + row) >> 1];
+        const float2 nyp = xbuf[(x + L * (yp + L * z)) >> 1];
+        const float2 nym = xbuf[(x + L * (ym + L * z)) >> 1];
+        // colour-1 neighbours in z: read back from the exchange
+        const int zn = z0 ? kn : kb, zq = z0 ? ka : kp;
+        const float2 nzp = xbuf[(x + L * (y + L * zn)) >> 1];
+        const float2 nzm = xbuf[(x + L * (y + L * zq)) >> 1];
+        const float4 jj = jt[z];
+        // bond (a -> a + e): cos = ca cb + sa sb, sin = sa cb - ca sb
+        e_c[0] += jj.x * ((cc * nxp.x + ss * nxp.y) + (nxm.x * cc + nxm.y * ss));
+        e_s[0] += jj.x * ((ss * nxp.x - cc * nxp.y) + (nxm.y * cc - nxm.x * ss));
+        e_c[1] += jj.x * ((cc * nyp.x + ss * nyp.y) + (nym.x * cc + nym.y * ss));
+        e_s[1] += jj.x * ((ss * nyp.x - cc * nyp.y) + (nym.y * cc - nym.x * ss));
+        e_c[2] += jj.y * (cc * nzp.x + ss * nzp.y) + jj.z * (nzm.x * cc + nzm.y * ss);
+        e_s[2] += jj.y * (ss * nzp.x - cc * nzp.y) + jj.z * (nzm.y * cc - nzm.x * ss);
+        mx += cc + nzp.x;  // the colour-0 site and its colour-1 z+ neighbour
+        my += ss + nzp.y;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    double v[9] = {-(double)(e_c[0] + e_c[1] + e_c[2]), (double)mx, (double)my,
+                   (double)e_c[0], (double)e_c[1], (double)e_c[2],
+                   (double)e_s[0], (double)e_s[1], (double)e_s[2]};
+    wg_sum<9, S::NW>(v, red);
+    if (t == 0) {
+      const double vol = (double)S::V;
+      const double e = v[0] / vol;
+      const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
+      const double vals[13] = {e,    e * e, m2,   m2 * m2,     v[3],        v[4],       v[5],
+                               v[6], v[7],  v[8], v[6] * v[6], v[7] * v[7], v[8] * v[8]};
+      double* acc = a.accum + (int64_t)rep * 13;
+      for (int k = 0; k < 13; ++k) acc[k] += vals[k];
+      int32_t* st = a.bin_state + 2 * rep;
+      st[0] += 1;
+      if (st[0] == a.binsize) {
+        if (st[1] < a.bin_cap)
+          for (int k = 0; k < 13; ++k)
+            a.bins[((int64_t)rep * 13 + k) * a.bin_cap + st[1]] = acc[k] / (double)a.binsize;
+        for (int k = 0; k < 13; ++k) acc[k] = 0.0;
+        st[0] = 0;
+        st[1] += 1;
+      }
+    }
+  };
+
+  auto wolff = [&]() __attribute__((always_inline)) {
+    asm volatile("" : "+v"(x), "+v"(y));
+    const Words ws = keyed(k0, k1, 0u, wctr, TAG_WSEED);
+    const float phit = u24(ws.w0);  // phi in turns
+    const int seed = (int)(((uint64_t)ws.w2 * (uint64_t)S::V) >> 32);
+    // r_i = cos(theta_i - phi) (src/updates.rs:579-582)
+    if (own) {
+#pragma unroll
+      for (int z = 0; z < L; ++z) rbuf[x + L * (y + L * z)] = cos_t(th[z] - phit);
+    }
+    __syncthreads();
+    const int wave = t >> 6, lane = t & 63;
+    const float m2b = -2.0f * beta_l2e;
+    const int xp = (x + 1 == L) ? 0 : x + 1, yp = (y + 1 == L) ? 0 : y + 1;
+    // bond decisions read r from LDS only, so this loop stays rolled
+#pragma unroll 1
+    for (int z = 0; z < L; ++z) {
+      const int zn = (z + 1 == L) ? 0 : z + 1;
+      bool bx_ = false, by_ = false, bz_ = false;
+      if (own) {
+        const int i = x + L * (y + L * z);
+        const float ri = rbuf[i];
+        const float rxp = rbuf[xp + L * (y + L * z)];
+        const float ryp = rbuf[x + L * (yp + L * z)];
+        const float rzp = rbuf[x + L * (y + L * zn)];
+        const float4 jj = jt[z];
+        const Words w = keyed(k0, k1, (uint32_t)i, wctr, TAG_WBOND);
+        const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
+        bx_ = (px > 0.0f) & (u24(w.w0) < 1.0f - exp2_hw(m2b * px));
+        by_ = (py > 0.0f) & (u24(w.w1) < 1.0f - exp2_hw(m2b * py));
+        bz_ = (pz > 0.0f) & (u24(w.w2) < 1.0f - exp2_hw(m2b * pz));
+      }
+      const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
+      if (lane < S::RPW) {
+        const int row = wave * S::RPW + lane + L * z;
+        const int sh = lane * S::RW;
+        bxm[row] = (uint32_t)(gx >> sh) & S::MASK;
+        bym[row] = (uint32_t)(gy >> sh) & S::MASK;
+        bzm[row] = (uint32_t)(gz >> sh) & S::MASK;
+      }
+    }
+    for (int r = t; r < S::ROWS; r += S::NT) vis[r] = (r == seed / L) ? (1u << (seed % L)) : 0u;
+    __syncthreads();
+    // flood fill to the fixed point; the flag of round r lives in slot r % 3,
+    // which thread 0 clears only in round r + 2, after every thread read it
+    for (int round = 0;; ++round) {
+      bool changed = false;
+      for (int r = t; r < S::ROWS; r += S::NT) {
+        const int ry = r % L, rz = r / L;
+        const int rowyp = ((ry + 1) % L) + L * rz, rowym = ((ry + L - 1) % L) + L * rz;
+        const int rowzp = ry + L * ((rz + 1) % L), rowzm = ry + L * ((rz + L - 1) % L);
+        const uint32_t v = vis[r];
+        uint32_t n = v | (vis[rowym] & bym[rowym]) | (vis[rowyp] & bym[r]) |
+                     (vis[rowzm] & bzm[rowzm]) | (vis[rowzp] & bzm[r]);
+        const uint32_t bx = bxm[r];
+        for (int it = 0; it < L; ++it) {
+          const uint32_t m = n | rotl1<L>(n & bx) | (rotr1<L>(n) & bx);
+          if (m == n) break;
+          n = m;
+        }
+        if (n != v) {
+          vis[r] = n;
+          changed = true;
+        }
+      }
+      if (changed) flags[round % 3] = 1;
+      if (t == 0) flags[(round + 1) % 3] = 0;
+      __syncthreads();
+      if (flags[round % 3] == 0) break;
+    }
+    // reflection theta -> 2 phi + pi - theta (src/updates.rs:503-506), in turns
+    if (own) {
+      const float refl = 2.0f * phit + 0.5f;
+#pragma unroll
+      for (int z = 0; z < L; ++z) {
+        const bool in = (vis[y + L * z] >> x) & 1u;
+        th[z] = in ? fract_hw(refl - th[z]) : th[z];
+      }
+    }
+    int members = 0;
+    for (int r = t; r < S::ROWS; r += S::NT) members += __popc(vis[r]);
+    ncluster += members;
+    wctr += 1u;
+  };
+
+  for (int sw = 0; sw < a.nsweeps; ++sw) {
+    exchange(1);
+    __syncthreads();
+    if (a.measure && sw > 0) measure();
+    update(0);
+    __syncthreads();
+    exchange(0);
+    __syncthreads();
+    update(1);
+    __syncthreads();
+    mctr += 1u;
+    for (int k = 0; k < d.wolff_steps; ++k) {
+      wolff();
+      __syncthreads();
+    }
+  }
+  if (a.measure && a.nsweeps > 0) {
+    exchange(1);
+    __syncthreads();
+    measure();
+  }
+
+  // write back theta in [0, 2 pi) and its trig cache, counters, tallies
+  if (own) {
+#pragma unroll
+    for (int z = 0; z < L; ++z) {
+      const int64_t g = base + x + L * (y + L * z);
+      float tr = th[z] * TWO_PI_F;
+      tr = tr >= TWO_PI_F ? tr - TWO_PI_F : tr;
+      a.theta[g] = tr;
+      a.cth[g] = cos_t(th[z]);
+      a.sth[g] = sin_t(th[z]);
+    }
+  }
+  double tallies[2] = {(t & 63) == 0 ? (double)nacc_wave : 0.0, (double)ncluster};
+  __syncthreads();
+  wg_sum<2, S::NW>(tallies, red);
+  if (t == 0) {
+    a.acc_out[rep] += (unsigned long long)tallies[0];
+    a.csize_out[rep] += (long long)tallies[1];
+    a.reps[rep].metro_ctr = mctr;
+    a.reps[rep].wolff_ctr = wctr;
+  }
+}
+
+__global__ void k_zero_tallies(unsigned long long* acc, long long* cs, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    acc[i] = 0ull;
+    cs[i] = 0ll;
+  }
+}
+
+bool resident_supports(int L) { return L == 8 || L == 12 || L == 16 || L == 24 || L == 32; }
+
+int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan) {
   plan.groups.clear();
-  return 0;
+  plan.covers_all = true;
+  std::map<int, std::vector<int32_t>> by_l;
+  for (size_t i = 0; i < reps.size(); ++i) {
+    const RepDev& r = reps[i];
+    if (r.lx == r.ly && r.ly == r.lz && resident_supports(r.lx)) by_l[r.lx].push_back((int32_t)i);
+    else plan.covers_all = false;
+  }
+  if (!plan.covers_all) return SLSF_OK;
+  for (auto& kv : by_l) {
+    ResidentGroup g;
+    g.L = kv.first;
+    g.replicas = kv.second;
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(int32_t) * g.replicas.size()) != hipSuccess)
+      return fail(SLSF_E_DEVICE, "hipMalloc failed for the resident plan");
+    if (hipMemcpy(p, g.replicas.data(), sizeof(int32_t) * g.replicas.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return fail(SLSF_E_DEVICE, "hipMemcpy failed for the resident plan");
+    g.d_index = (int32_t*)p;
+    plan.groups.push_back(std::move(g));
+  }
+  return SLSF_OK;
 }
-int resident_run(ResidentPlan&, const GlobalState&, const ResidentRunArgs&, hipStream_t,
-                 ResidentRunResult*) {
-  return SLSF_E_STATE;
+
+static int default_sweeps_per_launch(int L) { return L >= 24 ? 50 : 100; }
+
+template <int L>
+static void launch_resident(const ResArgs& a, int nblocks, bool paired, hipStream_t s) {
+  if (paired) k_resident<L, true><<<nblocks, RShape<L>::NT, 0, s>>>(a);
+  else k_resident<L, false><<<nblocks, RShape<L>::NT, 0, s>>>(a);
 }
+
+static void launch_any(int L, const ResArgs& a, int nblocks, bool paired, hipStream_t s) {
+  switch (L) {
+    case 8: launch_resident<8>(a, nblocks, paired, s); break;
+    case 12: launch_resident<12>(a, nblocks, paired, s); break;
+    case 16: launch_resident<16>(a, nblocks, paired, s); break;
+    case 24: launch_resident<24>(a, nblocks, paired, s); break;
+    default: launch_resident<32>(a, nblocks, paired, s); break;
+  }
+}
+
+int resident_run(ResidentPlan& plan, const GlobalState& g, const ResidentRunArgs& ra,
+                 hipStream_t s, ResidentRunResult* out) {
+  k_zero_tallies<<<(g.nrep + 255) / 256, 256, 0, s>>>(ra.acc_out, ra.csize_out, g.nrep);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(SLSF_E_DEVICE, "hipEventCreate failed");
+  double kernel_ms = 0.0;
+  int64_t launches = 0;
+  for (const ResidentGroup& grp : plan.groups) {
+    ResArgs a;
+    a.reps = g.reps;
+    a.index = grp.d_index;
+    a.theta = (float*)g.theta;
+    a.cth = (float*)g.cth;
+    a.sth = (float*)g.sth;
+    a.jxy = (const float*)g.jxy;
+    a.jz = (const float*)g.jz;
+    a.accum = ra.accum;
+    a.bins = ra.bins;
+    a.bin_state = ra.bin_state;
+    a.acc_out = ra.acc_out;
+    a.csize_out = ra.csize_out;
+    a.bin_cap = ra.bin_cap;
+    a.binsize = ra.binsize;
+    const int spl = ra.sweeps_per_launch > 0 ? ra.sweeps_per_launch
+                                             : default_sweeps_per_launch(grp.L);
+    const int nb = (int)grp.replicas.size();
+    for (int phase = 0; phase < 2; ++phase) {
+      int64_t left = phase == 0 ? ra.thermalization : ra.sweeps;
+      a.measure = phase;
+      while (left > 0) {
+        a.nsweeps = (int)std::min<int64_t>(left, spl);
+        hipEventRecord(e0, s);
+        launch_any(grp.L, a, nb, !ra.replay_stream, s);
+        hipEventRecord(e1, s);
+        if (hipGetLastError() != hipSuccess) return fail(SLSF_E_DEVICE, "resident launch failed");
+        if (hipEventSynchronize(e1) != hipSuccess)
+          return fail(SLSF_E_DEVICE, "resident kernel failed");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        kernel_ms += ms;
+        ++launches;
+        left -= a.nsweeps;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  out->kernel_ms = kernel_ms;
+  out->launches = launches;
+  return SLSF_OK;
+}
+
 }  // namespace slsf

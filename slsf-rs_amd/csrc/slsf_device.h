@@ -16,6 +16,9 @@
 //      tag WSEED: index = 0,    counter = cluster    -> phi u(w0,w1), seed site (w2*V)>>32
 //      tag WBOND: index = base site of bond (base, base+e_a), counter = cluster
 //                                                    -> bond uniform from word a
+//      tag METRO_PAIR (FP32 resident kernel, production stream): index = the
+//        first of two same-colour sites z, z+2 of a column, counter = sweep
+//                                -> (proposal, accept) = (w0, w1) and (w2, w3)
 //    FP64 uniforms use 53 bits (two words), FP32 uniforms the top 24 bits of the
 //    first word of the pair; bond uniforms are w*2^-32 (FP64) or (w>>8)*2^-24.
 #pragma once
@@ -31,6 +34,7 @@ constexpr uint32_t TAG_INIT = 1;
 constexpr uint32_t TAG_METRO = 2;
 constexpr uint32_t TAG_WSEED = 3;
 constexpr uint32_t TAG_WBOND = 4;
+constexpr uint32_t TAG_METRO_PAIR = 5;
 
 constexpr double TWO_PI = 3.141592653589793 * 2.0;  // src/types.rs:6
 constexpr double PI = 3.141592653589793;

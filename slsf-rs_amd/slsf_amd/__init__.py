@@ -60,7 +60,7 @@ class Observables(ctypes.Structure):
 class RunOptions(ctypes.Structure):
     _fields_ = [("thermalization", ctypes.c_int64), ("sweeps", ctypes.c_int64),
                 ("binsize", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("stream", ctypes.c_int32)]
 
 
 class RunStats(ctypes.Structure):
@@ -294,8 +294,10 @@ class Batch:
                 "corr_xy_by_z": cbz[:lz * rxy].reshape(lz, rxy), "corr_z": cz[:rz]}
 
     def run(self, thermalization: int, sweeps: int, binsize: int,
-            sweeps_per_launch: int = 0, use_graph: bool = False) -> dict:
-        opt = RunOptions(thermalization, sweeps, binsize, sweeps_per_launch, int(use_graph), 0)
+            sweeps_per_launch: int = 0, use_graph: bool = False,
+            replay_stream: bool = False) -> dict:
+        opt = RunOptions(thermalization, sweeps, binsize, sweeps_per_launch, int(use_graph),
+                         int(replay_stream))
         st = RunStats()
         _check(_lib.slsf_run(self._h, ctypes.byref(opt), ctypes.byref(st)))
         return {n: getattr(st, n) for n, _ in RunStats._fields_}

@@ -1,0 +1,176 @@
+"""GPU tests of the FP32 batched (register/LDS-resident) kernel.
+
+* Replay stream: the FP32 kernel consumes the FP64 replay path's per-site
+  uniforms; from the same state, one sweep differs only by FP32 rounding and
+  rare decisions taken the other way (reported per 10^6 updates).
+* Alone vs in a batch: a replica's chain is bit-identical whatever it is
+  batched with (counter-based RNG keyed by replica, per-workgroup arithmetic).
+* Statistics: energy of decoupled disordered z-rings against its exact Bessel
+  value; infinite temperature moments; FP32 production vs FP64 replay runs on
+  the same couplings within 4 combined sigma.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from test_oracle import _ring_energy
+
+pytestmark = pytest.mark.gpu
+slsf = pytest.importorskip("slsf_amd")
+
+
+def reps_for(L, n, T=1.3, seed0=100, jxy=None, jz=None, wolff=1, width=math.pi):
+    jxy = orc.uniform_layers(L, 0.6, 1.8, 7) if jxy is None else jxy
+    jz = np.ones(L) if jz is None else jz
+    return [slsf.Replica(L, L, L, T if np.isscalar(T) else T[i], jxy, jz, seed0 + i, width,
+                         wolff) for i in range(n)]
+
+
+def wrapped_diff(a, b):
+    d = np.abs(a - b) % (2 * math.pi)
+    return np.minimum(d, 2 * math.pi - d)
+
+
+@pytest.mark.parametrize("L", [8, 16, 32])
+def test_replay_stream_metropolis_decisions(L):
+    """Metropolis only (wolff_steps=0): decision mismatches per 1e6 updates
+    between the FP32 kernel and the FP64 replay kernel on the same uniforms."""
+    n = 8
+    reps = reps_for(L, n, wolff=0)
+    b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
+    b32 = slsf.Batch(reps, slsf.FP32)
+    b64.init_angles()
+    b32.init_angles()
+    b64.sweep(1)
+    b32.run(1, 0, 1, replay_stream=True)
+    bad = 0
+    maxd = 0.0
+    for r in range(n):
+        d = wrapped_diff(b64.get_theta(r), b32.get_theta(r))
+        mism = d > 1e-3
+        bad += int(mism.sum())
+        if (~mism).any():
+            maxd = max(maxd, float(d[~mism].max()))
+    rate = bad / (n * L ** 3) * 1e6
+    print(f"L={L}: decision mismatches per 1e6 updates = {rate:.2f}, max |dtheta| = {maxd:.2e}")
+    assert rate < 200.0
+    assert maxd < 2e-5
+
+
+@pytest.mark.parametrize("L", [8, 16])
+def test_replay_stream_full_sweep(L):
+    """Metropolis + Wolff: most replicas reproduce the FP64 sweep to FP32
+    rounding (a cluster flips differently only if a bond's uniform falls within
+    FP32 error of its probability)."""
+    n = 16
+    reps = reps_for(L, n, T=1.1)
+    b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
+    b32 = slsf.Batch(reps, slsf.FP32)
+    b64.init_angles()
+    b32.init_angles()
+    b64.sweep(1)
+    b32.run(1, 0, 1, replay_stream=True)
+    same = 0
+    for r in range(n):
+        d = wrapped_diff(b64.get_theta(r), b32.get_theta(r))
+        same += int(d.max() < 2e-5 or (d > 1e-3).mean() < 0.002)
+    assert same >= n - 2, same
+    m64, w64 = b64.counters()
+    m32, w32 = b32.counters()
+    assert list(m64) == list(m32) and list(w64) == list(w32)
+
+
+def test_alone_vs_batched_bit_identical():
+    L = 16
+    reps = reps_for(L, 5, T=[0.9, 1.2, 1.5, 2.0, 2.5])
+    big = slsf.Batch(reps, slsf.FP32)
+    one = slsf.Batch([reps[2]], slsf.FP32)
+    for b in (big, one):
+        b.init_angles()
+        b.run(15, 20, 5)
+    assert np.array_equal(big.get_theta(2), one.get_theta(0))
+    assert np.array_equal(big.all_bins(2), one.all_bins(0))
+
+
+def test_uneven_sizes_in_one_batch():
+    sizes = [8, 12, 16, 24, 32]
+    reps = []
+    for i, L in enumerate(sizes):
+        reps += reps_for(L, 2, T=1.2 + 0.05 * i, seed0=1000 * L)
+    big = slsf.Batch(reps, slsf.FP32)
+    big.init_angles()
+    st = big.run(10, 10, 5)
+    assert st["site_updates"] == sum(r.volume for r in reps) * 20
+    for i in (0, 3, 5, 9):
+        one = slsf.Batch([reps[i]], slsf.FP32)
+        one.init_angles()
+        one.run(10, 10, 5)
+        assert np.array_equal(big.get_theta(i), one.get_theta(0)), i
+        e = big.bins(i, "Energy")
+        assert len(e) == 2 and np.all(e < 0) and np.all(np.isfinite(e))
+
+
+def test_bins_match_observables_of_final_state():
+    L = 16
+    b = slsf.Batch(reps_for(L, 3), slsf.FP32)
+    b.init_angles()
+    b.run(5, 0, 1)
+    b.run(0, 1, 1)
+    obs = b.observables()
+    for r in range(3):
+        bins = b.all_bins(r)[:, -1]
+        o = obs[r]
+        assert abs(bins[0] - o["energy"]) < 2e-5 * abs(o["energy"]) + 1e-6
+        assert abs(bins[2] - o["magnetization_squared"]) < 2e-5
+        for k, name in zip(range(4, 10), ["cos_x", "cos_y", "cos_z", "sin_x", "sin_y", "sin_z"]):
+            assert abs(bins[k] - o[name]) < 2e-5 * L ** 3 * 0.05 + 1e-2, name
+
+
+def test_decoupled_rings_exact_energy():
+    """J_xy = 0: independent periodic z-rings with disordered J_z; the exact
+    <E> from Bessel functions (Z = sum_n prod_z I_n(beta J_z))."""
+    L = 16
+    jz = orc.uniform_layers(L, 0.4, 2.0, 11)
+    T = 0.9
+    n = 32
+    b = slsf.Batch(reps_for(L, n, T=T, jxy=np.zeros(L), jz=jz), slsf.FP32)
+    b.init_angles()
+    b.run(300, 2000, 100)
+    e = np.array([b.bins(r, "Energy").mean() for r in range(n)])
+    exact = _ring_energy(jz, 1.0 / T)
+    err = e.std(ddof=1) / math.sqrt(n)
+    print(f"rings: {e.mean():.6f} +- {err:.6f}, exact {exact:.6f}")
+    assert abs(e.mean() - exact) < 4 * err + 2e-5
+
+
+def test_infinite_temperature():
+    L = 8
+    n = 16
+    b = slsf.Batch(reps_for(L, n, T=1e7), slsf.FP32)
+    b.init_angles()
+    b.run(10, 2000, 100)
+    e = np.array([b.bins(r, "Energy").mean() for r in range(n)])
+    m2 = np.array([b.bins(r, "MagnetizationSquared").mean() for r in range(n)])
+    assert abs(e.mean()) < 4 * e.std(ddof=1) / math.sqrt(n) + 1e-4
+    assert abs(m2.mean() * L ** 3 - 1.0) < 0.05
+
+
+def test_fp32_statistics_match_fp64_replay():
+    """Layered disorder at L=8, T=1.3: FP32 production runs and FP64 replay
+    runs (independent streams) agree within 4 combined standard errors for the
+    energy, |m|^2 and the helicity sums."""
+    L, n, therm, sweeps = 8, 48, 100, 400
+    reps = reps_for(L, n, T=1.3, seed0=5000)
+    b32 = slsf.Batch(reps, slsf.FP32)
+    b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
+    for b in (b32, b64):
+        b.init_angles()
+        b.run(therm, sweeps, 100)
+    for name in ("Energy", "MagnetizationSquared", "_helicity_cos_x", "_helicity_cos_z"):
+        a = np.array([b32.bins(r, name).mean() for r in range(n)])
+        c = np.array([b64.bins(r, name).mean() for r in range(n)])
+        se = math.sqrt(a.var(ddof=1) / n + c.var(ddof=1) / n)
+        print(f"{name}: fp32 {a.mean():.6f} fp64 {c.mean():.6f} se {se:.2e}")
+        assert abs(a.mean() - c.mean()) < 4 * se, name
