@@ -95,6 +95,23 @@ def _cpu_worker(arg):
     return sweeps * 32 ** 3 / (time.perf_counter() - t0)
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Device-to-device copy of a known size (torch as plumbing): the
+    FETCH_SIZE / WRITE_SIZE counters of this dispatch calibrate the per-launch
+    HBM traffic the profiler reports for the lattice kernels."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_each_way": nbytes, "seconds": dt,
+                      "GB_per_s_read_plus_write": 2 * nbytes / dt / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,7 +121,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--realizations", type=int, default=256)
+    ap.add_argument("--sweeps-per-launch", type=int, default=0)
+    ap.add_argument("--mode", choices=["bench", "copy"], default="bench",
+                    help="copy: a plain 1 GiB device copy, to calibrate HBM byte counters")
     args = ap.parse_args()
+    if args.mode == "copy":
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the replica batch is sharded across GPUs "
               "by whole replicas, which this build does not launch", file=sys.stderr)
@@ -116,7 +139,7 @@ def main():
     b = slsf.Batch(reps, slsf.FP32)
     b.init_angles()
     b.run(args.warmup, 0, 1)
-    st = b.run(0, args.steps, args.binsize)
+    st = b.run(0, args.steps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
     V = 32 ** 3
     value = len(reps) * V * args.steps / st["seconds"]
     ms_per_step = 1e3 * st["seconds"] / args.steps
@@ -154,6 +177,11 @@ def main():
         "batched_over_plain": (value / plain) if plain else None,
         "acceptance": st["acceptance_mean"], "mean_cluster_size": st["mean_cluster_size"],
         "roofline": roofline, "cpu_baseline": cpu,
+        "kernel": {"name": "k_resident<32, paired>", "launches": st["launches"],
+                   "avg_launch_ms": kernel_ms / max(st["launches"], 1),
+                   "sweeps_per_launch": args.sweeps_per_launch or 50,
+                   "algorithmic_bytes_per_launch": len(reps) * V * bytes_per_update *
+                   (args.sweeps_per_launch or 50)},
     }
     print(json.dumps(out))
 
