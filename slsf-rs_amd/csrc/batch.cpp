@@ -342,11 +342,12 @@ int slsf_get_theta(slsf_batch* b, int32_t replica, double* theta_out) {
                            sizeof(double) * r.volume, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
   } else {
+    // FP32 state is kept in turns; report radians like the FP64 state
     std::vector<float> tmp(r.volume);
     HIP_TRY(hipMemcpyAsync(tmp.data(), (float*)b->g.theta + r.site_off,
                            sizeof(float) * r.volume, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    for (int64_t i = 0; i < r.volume; ++i) theta_out[i] = tmp[i];
+    for (int64_t i = 0; i < r.volume; ++i) theta_out[i] = (double)tmp[i] * (2.0 * M_PI);
   }
   return SLSF_OK;
 }
@@ -363,7 +364,13 @@ int slsf_set_theta(slsf_batch* b, int32_t replica, const double* theta) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     launch_refresh<double>(b->g, b->stream);
   } else {
-    std::vector<float> tmp(theta, theta + r.volume);
+    std::vector<float> tmp(r.volume);
+    for (int64_t i = 0; i < r.volume; ++i) {
+      double t = theta[i] / (2.0 * M_PI);
+      t -= std::floor(t);
+      float f = (float)t;
+      tmp[i] = f >= 1.0f ? 0.0f : f;
+    }
     HIP_TRY(hipMemcpyAsync((float*)b->g.theta + r.site_off, tmp.data(), sizeof(float) * r.volume,
                            hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));

@@ -148,12 +148,12 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
   const uint32_t k0 = d.key0, k1 = d.key1;
   uint32_t mctr = d.metro_ctr, wctr = d.wolff_ctr;
 
-  // angles of this thread's column in turns, [0, 1)
+  // angles of this thread's column in turns, [0, 1) (the FP32 state format)
   float th[L];
 #pragma unroll
   for (int z = 0; z < L; ++z) {
     th[z] = 0.0f;
-    if (own) th[z] = fract_hw(a.theta[base + x + L * (y + L * z)] * INV_TWO_PI_F);
+    if (own) th[z] = a.theta[base + x + L * (y + L * z)];
   }
   if (t < L) {
     const int zm = (t == 0) ? L - 1 : t - 1;
@@ -397,14 +397,12 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
     measure();
   }
 
-  // write back theta in [0, 2 pi) and its trig cache, counters, tallies
+  // write back the angles (turns) and their trig cache, counters, tallies
   if (own) {
 #pragma unroll
     for (int z = 0; z < L; ++z) {
       const int64_t g = base + x + L * (y + L * z);
-      float tr = th[z] * TWO_PI_F;
-      tr = tr >= TWO_PI_F ? tr - TWO_PI_F : tr;
-      a.theta[g] = tr;
+      a.theta[g] = th[z];
       a.cth[g] = cos_t(th[z]);
       a.sth[g] = sin_t(th[z]);
     }

@@ -174,3 +174,42 @@ def test_fp32_statistics_match_fp64_replay():
         se = math.sqrt(a.var(ddof=1) / n + c.var(ddof=1) / n)
         print(f"{name}: fp32 {a.mean():.6f} fp64 {c.mean():.6f} se {se:.2e}")
         assert abs(a.mean() - c.mean()) < 4 * se, name
+
+
+def test_launch_split_invariance():
+    """The same run split into launches of 7 or 64 sweeps is bit-identical:
+    the FP32 state (angles in turns) is stored exactly between launches."""
+    L = 16
+    reps = reps_for(L, 3)
+    out = []
+    for spl in (7, 64):
+        b = slsf.Batch(reps, slsf.FP32)
+        b.init_angles()
+        b.run(20, 30, 5, sweeps_per_launch=spl)
+        out.append((np.stack([b.get_theta(r) for r in range(3)]),
+                    np.stack([b.all_bins(r) for r in range(3)])))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_checkpoint_resume_equivalence(precision):
+    """N sweeps == k sweeps, checkpoint (theta + counters), restore into a new
+    handle, N - k sweeps (the analogue of ThetaModel's serde checkpoint)."""
+    prec = slsf.FP32 if precision == "fp32" else slsf.FP64_REPLAY
+    L = 8
+    rep = reps_for(L, 1, T=1.2)
+    a = slsf.Batch(rep, prec)
+    a.init_angles()
+    a.run(30, 0, 1)
+    b = slsf.Batch(rep, prec)
+    b.init_angles()
+    b.run(12, 0, 1)
+    theta = b.get_theta(0)
+    m, w = b.counters()
+    c = slsf.Batch(rep, prec)
+    c.set_theta(0, theta)
+    c.set_counters(int(m[0]), int(w[0]))
+    c.run(18, 0, 1)
+    assert np.array_equal(a.get_theta(0), c.get_theta(0))
+    assert list(a.counters()[0]) == list(c.counters()[0]) == [30]
