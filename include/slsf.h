@@ -138,6 +138,44 @@ int slsf_measure_correlations(slsf_batch* batch, int32_t replica, int32_t rmax,
                               double* corr_z);
 
 /* ======================================================================== */
+/* Plain driver                                                              */
+/* ======================================================================== */
+
+/* ThetaSimulationOptions (src/types.rs:265-296); rmax < 0 means None. */
+typedef struct {
+  int32_t thermal_sweeps, measure_sweeps, measure_interval;
+  double proposal_width;
+  int32_t wolff_steps, correlation_interval;
+  int32_t correlation_rmax, correlation_rmax_xy, correlation_rmax_z;
+  int32_t cold_init;
+} slsf_simulation_options;
+
+/* ThetaSimulationResult (src/types.rs:298-320).  The correlation vectors are
+ * caller buffers with capacity >= l_x/2 (corr_x, corr_y, corr_xy), l_z * l_x/2
+ * (corr_xy_by_z, [z][r-1]) and l_z/2 (corr_z); rxy / rz receive their lengths. */
+typedef struct {
+  double energy, rho_sx, rho_sy, rho_sz, std_rho_sx, std_rho_sy, std_rho_sz;
+  double magnetization_squared, chi, acceptance;
+  int32_t num_measurements, num_correlation_measurements, rxy, rz;
+  double* corr_x;
+  double* corr_y;
+  double* corr_xy;
+  double* corr_xy_by_z;
+  double* corr_z;
+} slsf_simulation_result;
+
+/* Replaces run_theta_simulation (src/simulation.rs:13-195) on the GPU.  Same
+ * validation order and messages.  Disorder: j_xy_array / j_z_array when given
+ * (length l_z), else the reference's initialize_disorder (uniform J_xy, two-point
+ * J_z) drawn from SplitMix64(seed).  Angles and updates use the keyed uniforms of
+ * `seed`.  precision: SLSF_FP64_REPLAY or SLSF_FP32. */
+int slsf_run_theta_simulation(int32_t l_x, int32_t l_y, int32_t l_z, double j_xy,
+                              double j_z_mean, double delta_j_z, double temperature,
+                              const double* j_xy_array, const double* j_z_array,
+                              const slsf_simulation_options* options, uint64_t seed,
+                              int32_t precision, slsf_simulation_result* result);
+
+/* ======================================================================== */
 /* Production runs with on-device binning                                    */
 /* ======================================================================== */
 

@@ -797,9 +797,50 @@ static double mean_square(const double* x, int n) {
   return s / (double)n;
 }
 
+static int run_sim(or_lattice* l, double j_xy, double j_z_mean, double delta_j_z,
+                   double temperature, const double* jxy_arr, const double* jz_arr,
+                   const or_sim_options* o, or_fastrng* rng, int replay, uint64_t seed,
+                   or_sim_result* res);
+
 int or_run_theta_simulation(or_lattice* l, double j_xy, double j_z_mean, double delta_j_z,
                             double temperature, const or_sim_options* o, or_fastrng* rng,
                             or_sim_result* res) {
+  return run_sim(l, j_xy, j_z_mean, delta_j_z, temperature, NULL, NULL, o, rng, 0, 0, res);
+}
+
+int or_run_theta_simulation_replay(or_lattice* l, double j_xy, double j_z_mean,
+                                   double delta_j_z, double temperature,
+                                   const double* jxy_arr, const double* jz_arr,
+                                   const or_sim_options* o, uint64_t seed, or_sim_result* res) {
+  or_fastrng rng;
+  or_rng_seed(&rng, seed);
+  return run_sim(l, j_xy, j_z_mean, delta_j_z, temperature, jxy_arr, jz_arr, o, &rng, 1, seed,
+                 res);
+}
+
+/* one sweep of the driver: Metropolis + wolff_steps clusters, either mode */
+static int driver_sweep(or_lattice* l, double temperature, const or_sim_options* o,
+                        or_fastrng* rng, int replay, uint64_t seed, uint32_t* metro_ctr,
+                        uint32_t* wolff_ctr, double* a) {
+  int64_t cs;
+  if (replay) {
+    if (or_metropolis_sweep_replay(l, temperature, o->proposal_width, seed, (*metro_ctr)++,
+                                   NULL, a)) return 1;
+    for (int k = 0; k < o->wolff_steps; ++k)
+      if (or_wolff_step_replay(l, temperature, seed, (*wolff_ctr)++, NULL, &cs)) return 1;
+  } else {
+    if (or_metropolis_sweep_ref(l, temperature, o->proposal_width, o->use_batch8, rng, a))
+      return 1;
+    for (int k = 0; k < o->wolff_steps; ++k)
+      if (or_wolff_step_ref(l, temperature, 1, rng, &cs)) return 1;
+  }
+  return 0;
+}
+
+static int run_sim(or_lattice* l, double j_xy, double j_z_mean, double delta_j_z,
+                   double temperature, const double* jxy_arr, const double* jz_arr,
+                   const or_sim_options* o, or_fastrng* rng, int replay, uint64_t seed,
+                   or_sim_result* res) {
   if (o->measure_interval <= 0) return fail("measure_interval must be positive");
   if (o->correlation_interval <= 0) return fail("correlation_interval must be positive");
   int nmeas = o->measure_sweeps / o->measure_interval;
@@ -810,20 +851,27 @@ int or_run_theta_simulation(or_lattice* l, double j_xy, double j_z_mean, double 
   int rxy, rz;
   or_correlation_rmax(l, o->correlation_rmax, o->correlation_rmax_xy, o->correlation_rmax_z,
                       &rxy, &rz);
-  if (or_initialize_disorder(l, j_xy, j_z_mean, delta_j_z, rng)) return 1;
+  if (jxy_arr && jz_arr) {
+    for (int z = 0; z < l->lz; ++z) { l->j_xy[z] = jxy_arr[z]; l->j_z[z] = jz_arr[z]; }
+  } else if (or_initialize_disorder(l, j_xy, j_z_mean, delta_j_z, rng)) {
+    return 1;
+  }
   for (int z = 0; z < l->lz; ++z)
     if (l->j_xy[z] < 0.0 || l->j_z[z] < 0.0)
       return fail("theta simulation requires nonnegative layer couplings");
-  or_initialize_angles(l, o->cold_init, rng);
-  or_scratch_refresh(l, 0);
+  if (replay) {
+    if (o->cold_init) { for (size_t i = 0; i < vol(l); ++i) l->theta[i] = 0.0; }
+    else or_initialize_angles_keyed(l, seed);
+  } else {
+    or_initialize_angles(l, o->cold_init, rng);
+  }
+  or_scratch_refresh(l, replay);
   double acc_sum = 0.0, a;
   int acc_count = 0;
-  int64_t cs;
+  uint32_t metro_ctr = 0, wolff_ctr = 0;
   for (int t = 0; t < o->thermal_sweeps; ++t) {
-    if (or_metropolis_sweep_ref(l, temperature, o->proposal_width, o->use_batch8, rng, &a)) return 1;
+    if (driver_sweep(l, temperature, o, rng, replay, seed, &metro_ctr, &wolff_ctr, &a)) return 1;
     acc_sum += a; acc_count++;
-    for (int k = 0; k < o->wolff_steps; ++k)
-      if (or_wolff_step_ref(l, temperature, 1, rng, &cs)) return 1;
   }
   double* series = calloc((size_t)nmeas * 8, sizeof(double));
   double *en = series, *cxs = series + nmeas, *cys = series + 2 * nmeas,
@@ -842,10 +890,8 @@ int or_run_theta_simulation(or_lattice* l, double j_xy, double j_z_mean, double 
   double* tcz = calloc((size_t)rz + 1, sizeof(double));
   int nm = 0, ncorr = 0;
   for (int step = 1; step <= o->measure_sweeps; ++step) {
-    if (or_metropolis_sweep_ref(l, temperature, o->proposal_width, o->use_batch8, rng, &a)) return 1;
+    if (driver_sweep(l, temperature, o, rng, replay, seed, &metro_ctr, &wolff_ctr, &a)) return 1;
     acc_sum += a; acc_count++;
-    for (int k = 0; k < o->wolff_steps; ++k)
-      if (or_wolff_step_ref(l, temperature, 1, rng, &cs)) return 1;
     if (step % o->measure_interval == 0) {
       or_observables ob;
       or_measure_observables(l, &ob);

@@ -148,6 +148,15 @@ int or_run_theta_simulation(or_lattice* lat, double j_xy, double j_z_mean,
                             const or_sim_options* opt, or_fastrng* rng,
                             or_sim_result* res);
 
+/* Replay-mode driver: the same flow and estimators as or_run_theta_simulation,
+ * with disorder from explicit arrays (when non-NULL) or FastRng(seed), angles
+ * and updates from keyed uniforms of `seed` (the GPU driver's contract). */
+int or_run_theta_simulation_replay(or_lattice* lat, double j_xy, double j_z_mean,
+                                   double delta_j_z, double temperature,
+                                   const double* j_xy_array, const double* j_z_array,
+                                   const or_sim_options* opt, uint64_t seed,
+                                   or_sim_result* res);
+
 /* ---- estimators -------------------------------------------------------- */
 double or_mean(const double* x, int n);
 double or_sample_std(const double* x, int n);

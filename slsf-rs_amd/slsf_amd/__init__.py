@@ -85,6 +85,24 @@ class Task(ctypes.Structure):
                 ("correlation_interval", ctypes.c_int32), ("disorder_kind", ctypes.c_int32)]
 
 
+class SimOptions(ctypes.Structure):
+    _fields_ = [("thermal_sweeps", ctypes.c_int32), ("measure_sweeps", ctypes.c_int32),
+                ("measure_interval", ctypes.c_int32), ("proposal_width", ctypes.c_double),
+                ("wolff_steps", ctypes.c_int32), ("correlation_interval", ctypes.c_int32),
+                ("correlation_rmax", ctypes.c_int32), ("correlation_rmax_xy", ctypes.c_int32),
+                ("correlation_rmax_z", ctypes.c_int32), ("cold_init", ctypes.c_int32)]
+
+
+class SimResult(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "energy", "rho_sx", "rho_sy", "rho_sz", "std_rho_sx", "std_rho_sy", "std_rho_sz",
+        "magnetization_squared", "chi", "acceptance")] + [
+        ("num_measurements", ctypes.c_int32), ("num_correlation_measurements", ctypes.c_int32),
+        ("rxy", ctypes.c_int32), ("rz", ctypes.c_int32)] + [
+        (n, ctypes.POINTER(ctypes.c_double)) for n in (
+            "corr_x", "corr_y", "corr_xy", "corr_xy_by_z", "corr_z")]
+
+
 _P = ctypes.POINTER
 _dp = _P(ctypes.c_double)
 _u8p = _P(ctypes.c_uint8)
@@ -137,6 +155,11 @@ _SIGS = {
     "slsf_job_selected_tasks": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int32,
                                                  ctypes.c_int32, _i64p, ctypes.c_int64]),
     "slsf_parse_duration": (ctypes.c_int, [ctypes.c_char_p, _i64p]),
+    "slsf_run_theta_simulation": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                                 _P(SimOptions), ctypes.c_uint64,
+                                                 ctypes.c_int32, _P(SimResult)]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
@@ -320,6 +343,39 @@ class Batch:
         out = np.zeros(len(self))
         _check(_lib.slsf_get_acceptance(self._h, _ptr(out, ctypes.c_double)))
         return out
+
+
+# ---- plain driver (src/simulation.rs:13) --------------------------------------
+def run_theta_simulation(l_x, l_y, l_z, j_xy, j_z_mean, delta_j_z, temperature, *,
+                         thermal_sweeps=1000, measure_sweeps=5000, measure_interval=10,
+                         proposal_width=float(np.pi), wolff_steps=1, correlation_interval=1,
+                         correlation_rmax=None, correlation_rmax_xy=None,
+                         correlation_rmax_z=None, cold_init=False, seed=0,
+                         j_xy_array=None, j_z_array=None, precision=FP64_REPLAY) -> dict:
+    """GPU run_theta_simulation; defaults are ThetaSimulationOptions::default
+    (src/types.rs:280-296)."""
+    none = lambda v: -1 if v is None else int(v)
+    o = SimOptions(thermal_sweeps, measure_sweeps, measure_interval, proposal_width,
+                   wolff_steps, correlation_interval, none(correlation_rmax),
+                   none(correlation_rmax_xy), none(correlation_rmax_z), int(cold_init))
+    cap = max(1, max(l_x, l_y, 1) // 2)
+    bufs = [np.zeros(cap), np.zeros(cap), np.zeros(cap), np.zeros(max(1, l_z) * cap),
+            np.zeros(max(1, l_z // 2))]
+    r = SimResult()
+    r.corr_x, r.corr_y, r.corr_xy, r.corr_xy_by_z, r.corr_z = (_ptr(b, ctypes.c_double)
+                                                               for b in bufs)
+    ax = None if j_xy_array is None else np.ascontiguousarray(j_xy_array, dtype=np.float64)
+    az = None if j_z_array is None else np.ascontiguousarray(j_z_array, dtype=np.float64)
+    _check(_lib.slsf_run_theta_simulation(l_x, l_y, l_z, j_xy, j_z_mean, delta_j_z, temperature,
+                                          _ptr(ax, ctypes.c_double), _ptr(az, ctypes.c_double),
+                                          ctypes.byref(o), seed & 0xFFFFFFFFFFFFFFFF,
+                                          precision, ctypes.byref(r)))
+    out = {n: getattr(r, n) for n, t in SimResult._fields_ if t in (ctypes.c_double,
+                                                                    ctypes.c_int32)}
+    rxy, rz = r.rxy, r.rz
+    out.update(corr_x=bufs[0][:rxy], corr_y=bufs[1][:rxy], corr_xy=bufs[2][:rxy],
+               corr_xy_by_z=bufs[3][:l_z * rxy].reshape(l_z, rxy), corr_z=bufs[4][:rz])
+    return out
 
 
 # ---- estimators --------------------------------------------------------------

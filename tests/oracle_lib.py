@@ -114,6 +114,11 @@ def load(fast: bool = False):
                                                    ctypes.c_double, ctypes.c_double,
                                                    ctypes.c_double, _P(OSimOptions),
                                                    _P(FastRng), _P(OSimResult)]),
+        "or_run_theta_simulation_replay": (ctypes.c_int, [_P(OLattice), ctypes.c_double,
+                                                          ctypes.c_double, ctypes.c_double,
+                                                          ctypes.c_double, _dp, _dp,
+                                                          _P(OSimOptions), ctypes.c_uint64,
+                                                          _P(OSimResult)]),
         "or_mean": (ctypes.c_double, [_dp, ctypes.c_int]),
         "or_sample_std": (ctypes.c_double, [_dp, ctypes.c_int]),
         "or_blocking_stderr": (ctypes.c_double, [_dp, ctypes.c_int]),

@@ -181,3 +181,62 @@ def test_run_bins_match_oracle_series():
     assert np.allclose(got, expect, rtol=1e-11, atol=1e-12)
 
 
+
+
+def _oracle_driver(dims, T, jxy, jz, seed, **opt):
+    lat = orc.Lattice(*dims)
+    o = orc.OSimOptions(opt.get("thermal_sweeps", 1000), opt.get("measure_sweeps", 5000),
+                        opt.get("measure_interval", 10), opt.get("proposal_width", np.pi),
+                        opt.get("wolff_steps", 1), opt.get("correlation_interval", 1),
+                        opt.get("correlation_rmax", -1), -1, -1, int(opt.get("cold_init", 0)), 0)
+    res = orc.OSimResult()
+    bufs = [np.zeros(64) for _ in range(5)]
+    res.corr_x, res.corr_y, res.corr_xy, res.corr_xy_by_z, res.corr_z = (
+        b.ctypes.data_as(orc._dp) for b in bufs)
+    import ctypes
+    rc = orc.lib().or_run_theta_simulation_replay(
+        lat.ref, 1.0, 1.0, 0.0, T, jxy.ctypes.data_as(orc._dp), jz.ctypes.data_as(orc._dp),
+        ctypes.byref(o), seed, ctypes.byref(res))
+    assert rc == 0, orc.err()
+    return res, bufs
+
+
+def test_driver_c1_matches_oracle_replay():
+    """The plain driver on the GPU (FP64 replay) against the oracle's replay
+    driver: L=16, T=1.3, J_xy[z] ~ U[0.6,1.8], J_perp = 1, C(r) up to r=8,
+    measured every sweep.  Every estimator agrees to 1e-10 (relative)."""
+    L, T, seed = 16, 1.3, 1
+    jxy = orc.uniform_layers(L, 0.6, 1.8, 1)
+    jz = np.ones(L)
+    opt = dict(thermal_sweeps=50, measure_sweeps=150, measure_interval=1, correlation_rmax=8)
+    g = slsf.run_theta_simulation(L, L, L, 1.0, 1.0, 0.0, T, seed=seed, j_xy_array=jxy,
+                                  j_z_array=jz, **opt)
+    o, bufs = _oracle_driver((L, L, L), T, jxy, jz, seed, **opt)
+    for k in ("energy", "rho_sx", "rho_sy", "rho_sz", "std_rho_sx", "std_rho_sy", "std_rho_sz",
+              "magnetization_squared", "chi", "acceptance"):
+        ov = getattr(o, k)
+        assert abs(g[k] - ov) <= 1e-10 * max(1.0, abs(ov)), k
+    assert g["num_measurements"] == o.num_measurements == 150
+    assert g["num_correlation_measurements"] == o.num_correlation_measurements
+    assert np.allclose(g["corr_x"], bufs[0][:8], atol=1e-12)
+    assert np.allclose(g["corr_z"], bufs[4][:8], atol=1e-12)
+    assert np.allclose(g["corr_xy_by_z"].ravel(), bufs[3][:L * 8], atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_driver_reference_assertions(precision):  # src/tests.rs:241-298,333-353
+    r = slsf.run_theta_simulation(4, 4, 4, 1.0, 1.0, 0.0, 0.5, thermal_sweeps=5,
+                                  measure_sweeps=10, measure_interval=2, proposal_width=0.4,
+                                  correlation_rmax=1, cold_init=True, seed=14,
+                                  precision=precision)
+    for k in ("energy", "rho_sx", "rho_sy", "rho_sz", "chi"):
+        assert np.isfinite(r[k])
+    assert 0 <= r["magnetization_squared"] <= 1 and 0 <= r["acceptance"] <= 1
+    assert len(r["corr_x"]) == 1 and len(r["corr_z"]) == 1
+    assert r["num_correlation_measurements"] == r["num_measurements"] == 5
+    t = slsf.run_theta_simulation(2, 2, 2, 1.0, 1.0, 0.0, 0.5, thermal_sweeps=0,
+                                  measure_sweeps=5, measure_interval=1, proposal_width=0.0,
+                                  wolff_steps=0, correlation_rmax=1, correlation_interval=2,
+                                  cold_init=True, precision=precision)
+    assert t["num_measurements"] == 5 and t["num_correlation_measurements"] == 3
+    assert abs(t["corr_x"][0] - 1.0) < 1e-6

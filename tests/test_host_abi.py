@@ -299,3 +299,32 @@ def test_jackknife_closed_forms():
     loo = (e.sum() - e) / (nb - 1)
     est, err = slsf.jackknife(loo, e.mean())
     assert abs(est - e.mean()) < 1e-12 and abs(err - e.std(ddof=1) / math.sqrt(nb)) < 1e-12
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(measure_interval=0), "measure_interval must be positive"),
+    (dict(thermal_sweeps=0, measure_sweeps=1, measure_interval=2),
+     "measure_sweeps must include at least one measurement; require measure_sweeps >= "
+     "measure_interval"),
+    (dict(correlation_interval=0), "correlation_interval must be positive"),
+    (dict(thermal_sweeps=0, measure_sweeps=1, measure_interval=1, proposal_width=math.nan),
+     "proposal_width must be nonnegative and finite"),
+])
+def test_driver_validation_messages(kw, msg):  # src/tests.rs:300-352
+    with pytest.raises(slsf.SlsfError) as e:
+        slsf.run_theta_simulation(2, 2, 2, 1.0, 1.0, 0.0, 0.5, **kw)
+    assert e.value.message == msg
+
+
+def test_driver_disorder_messages():  # src/tests.rs:354-369
+    with pytest.raises(slsf.SlsfError) as e:
+        slsf.run_theta_simulation(2, 2, 2, 1.0, -0.1, 0.0, 1.0, thermal_sweeps=0,
+                                  measure_sweeps=1, measure_interval=1)
+    assert e.value.message == "two-point layer disorder requires J_z_mean - δJ_z >= 0"
+    with pytest.raises(slsf.SlsfError) as e:
+        slsf.run_theta_simulation(2, 2, 2, 1.0, 1.0, -0.5, 1.0, thermal_sweeps=0,
+                                  measure_sweeps=1, measure_interval=1)
+    assert e.value.message == "δJ_z must be nonnegative"
+    with pytest.raises(slsf.SlsfError) as e:
+        slsf.run_theta_simulation(2, 2, 2, 1.0, 1.0, 0.0, 0.0)
+    assert e.value.message == "temperature T must be positive and finite"
