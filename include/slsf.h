@@ -227,6 +227,24 @@ typedef struct {
  * of every series on the device.  May be called repeatedly; bins append. */
 int slsf_run(slsf_batch* batch, const slsf_run_options* opt, slsf_run_stats* stats);
 
+/* Autotuner of the FP32 batched kernel.  For every replica size of the batch
+ * it times each candidate (z-columns per thread, i.e. waves per replica, and
+ * sweeps per launch) on the batch's own replicas, keeps the fastest, and
+ * restores the replicas' state and counters, so tuning does not change any
+ * chain.  records (may be NULL) receives up to cap trial records; *count the
+ * number of trials. */
+typedef struct {
+  int32_t l;                  /* replica edge of the group */
+  int32_t columns_per_thread; /* 1 or 2 */
+  int32_t waves_per_replica;
+  int32_t sweeps_per_launch;
+  double site_updates_per_s;  /* trial rate */
+  int32_t chosen;             /* 1 for the configuration kept */
+  int32_t reserved;
+} slsf_tune_record;
+int slsf_autotune(slsf_batch* batch, int32_t trial_sweeps, slsf_tune_record* records,
+                  int32_t cap, int32_t* count);
+
 /* Completed bins of one series of one replica: writes min(cap, nbins) values
  * and returns the count in *nbins. */
 int slsf_get_bins(slsf_batch* batch, int32_t replica, int32_t series, double* out,

@@ -72,6 +72,7 @@ struct slsf_batch {
   bool initialised = false;
 
   ~slsf_batch() {
+    resident_release(resident);
     for (void* p : allocations) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -712,6 +713,37 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     stats->acceptance_mean = acc / (double)b->reps.size();
     stats->mean_cluster_size = ncl ? cl / (double)ncl : 0.0;
   }
+  return SLSF_OK;
+}
+
+int slsf_autotune(slsf_batch* b, int32_t trial_sweeps, slsf_tune_record* records, int32_t cap,
+                  int32_t* count) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (count) *count = 0;
+  if (b->precision != SLSF_FP32 || !b->resident.covers_all)
+    return SLSF_OK;  // nothing to tune: the HBM kernels have no candidates
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  std::vector<TuneRecord> recs;
+  if (int rc = resident_autotune(b->resident, b->g, b->stream, std::max(trial_sweeps, 1), recs))
+    return rc;
+  int32_t n = 0;
+  for (const auto& r : recs) {
+    if (records && n < cap) {
+      slsf_tune_record& o = records[n];
+      o.l = r.L;
+      o.columns_per_thread = r.cpt;
+      o.waves_per_replica = r.waves;
+      o.sweeps_per_launch = r.spl;
+      o.site_updates_per_s = r.rate;
+      o.chosen = 0;
+      o.reserved = 0;
+      for (const auto& grp : b->resident.groups)
+        if (grp.L == r.L && grp.cpt == r.cpt && grp.sweeps_per_launch == r.spl) o.chosen = 1;
+    }
+    ++n;
+  }
+  if (count) *count = n;
   return SLSF_OK;
 }
 

@@ -12,6 +12,8 @@ namespace slsf {
 
 struct ResidentGroup {
   int L = 0;                    // cubic edge of every replica in the group
+  int cpt = 1;                  // z-columns per thread (waves per replica = L*RW/64/cpt)
+  int sweeps_per_launch = 0;    // 0: default
   std::vector<int32_t> replicas;  // batch indices
   int32_t* d_index = nullptr;   // device copy of `replicas`
 };
@@ -19,6 +21,14 @@ struct ResidentGroup {
 struct ResidentPlan {
   bool covers_all = false;      // every replica is cubic with a resident L
   std::vector<ResidentGroup> groups;
+  std::vector<hipStream_t> streams;  // one per group after the first
+  std::vector<hipEvent_t> events;    // join events of those streams
+  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+};
+
+struct TuneRecord {
+  int L, cpt, spl, waves;
+  double rate;  // site updates per second of the trial
 };
 
 struct ResidentRunArgs {
@@ -43,5 +53,10 @@ bool resident_supports(int L);
 int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan);
 int resident_run(ResidentPlan& plan, const GlobalState& g, const ResidentRunArgs& a,
                  hipStream_t s, ResidentRunResult* out);
+void resident_release(ResidentPlan& plan);
+int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
+                      int trial_sweeps, std::vector<TuneRecord>& records);
+std::vector<int> resident_cpt_candidates(int L);
+int resident_waves_per_replica(int L, int cpt);
 
 }  // namespace slsf

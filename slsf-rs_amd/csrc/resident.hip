@@ -43,10 +43,11 @@
 
 namespace slsf {
 
-template <int L>
+template <int L, int CPT>
 struct RShape {
   static constexpr int RW = L <= 8 ? 8 : (L <= 16 ? 16 : 32);
-  static constexpr int NT = RW * L;
+  static constexpr int YS = L / CPT;  // y stride between a thread's columns
+  static constexpr int NT = RW * YS;
   static constexpr int V = L * L * L;
   static constexpr int ROWS = L * L;
   static constexpr int NW = NT / 64;
@@ -54,6 +55,7 @@ struct RShape {
   static constexpr uint32_t MASK = (L == 32) ? 0xffffffffu : ((1u << L) - 1u);
   static_assert(NT % 64 == 0, "workgroup must be whole waves");
   static_assert((L / 2) % 2 == 0, "pairs of same-colour sites per column");
+  static_assert(YS % 2 == 0, "a thread's columns share their colour parity");
 };
 
 struct ResArgs {
@@ -74,7 +76,7 @@ struct ResArgs {
 
 template <int L>
 SLSF_D uint32_t rotl1(uint32_t w) {
-  return ((w << 1) | (w >> (L - 1))) & RShape<L>::MASK;
+  return ((w << 1) | (w >> (L - 1))) & RShape<L, 1>::MASK;
 }
 template <int L>
 SLSF_D uint32_t rotr1(uint32_t w) {
@@ -127,9 +129,9 @@ SLSF_D void wg_sum(double (&v)[N], double* red) {
 // stream, tag METRO_PAIR, counter index = first site of the pair); otherwise
 // one call per site with the replay path's counters (tag METRO), which lets an
 // FP32 run consume exactly the uniforms the FP64 replay path uses.
-template <int L, bool PAIRED>
-__global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
-  using S = RShape<L>;
+template <int L, int CPT, bool PAIRED>
+__global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
+  using S = RShape<L, CPT>;
   __shared__ float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V]
   __shared__ uint32_t bxm[S::ROWS], bym[S::ROWS], bzm[S::ROWS], vis[S::ROWS];
   __shared__ float4 jt[L];           // {J_xy[z], J_z[z], J_z[z-1], 0}
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
   const int rep = a.index[blockIdx.x];
   const RepDev d = a.reps[rep];
   const int t = threadIdx.x;
-  int x = t % S::RW, y = t / S::RW;
+  int x = t % S::RW, y = t / S::RW;  // column j of this thread is (x, y + j * YS)
   const bool own = x < L;
   const int64_t base = d.site_off;
   const float beta_l2e = (float)(d.beta * 1.4426950408889634);
@@ -149,12 +151,14 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
   uint32_t mctr = d.metro_ctr, wctr = d.wolff_ctr;
 
   // angles of this thread's column in turns, [0, 1) (the FP32 state format)
-  float th[L];
+  float th[CPT][L];
 #pragma unroll
-  for (int z = 0; z < L; ++z) {
-    th[z] = 0.0f;
-    if (own) th[z] = a.theta[base + x + L * (y + L * z)];
-  }
+  for (int j = 0; j < CPT; ++j)
+#pragma unroll
+    for (int z = 0; z < L; ++z) {
+      th[j][z] = 0.0f;
+      if (own) th[j][z] = a.theta[base + x + L * (y + j * S::YS + L * z)];
+    }
   if (t < L) {
     const int zm = (t == 0) ? L - 1 : t - 1;
     jt[t] = make_float4(a.jxy[d.jz_off + t], a.jz[d.jz_off + t], a.jz[d.jz_off + zm], 0.0f);
@@ -171,26 +175,32 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
     const int z1 = (col + x + y) & 1;
     if (!own) return;
 #pragma unroll
-    for (int k = 0; k < L / 2; ++k) {
-      const int z = 2 * k + z1;
-      const float tv = sel(z1, th[2 * k + 1], th[2 * k]);
-      xbuf[(x + L * (y + L * z)) >> 1] = make_float2(cos_t(tv), sin_t(tv));
-    }
+    for (int j = 0; j < CPT; ++j)
+#pragma unroll
+      for (int k = 0; k < L / 2; ++k) {
+        const int z = 2 * k + z1;
+        const float tv = sel(z1, th[j][2 * k + 1], th[j][2 * k]);
+        xbuf[(x + L * (y + j * S::YS + L * z)) >> 1] = make_float2(cos_t(tv), sin_t(tv));
+      }
   };
 
   // Metropolis update of this thread's colour-p sites
   auto update = [&](int p) __attribute__((always_inline)) {
     asm volatile("" : "+v"(x), "+v"(y));
+    const int y0 = y;
     const int z0 = (p + x + y) & 1;
     if (!own) return;
     const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
-    const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
     Words w{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+    const int y = y0 + j * S::YS;
+    const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
 #pragma unroll
     for (int k = 0; k < L / 2; ++k) {
       const int z = 2 * k + z0;
       const int ka = 2 * k, kb = 2 * k + 1;
-      const float t0 = sel(z0, th[kb], th[ka]);
+      const float t0 = sel(z0, th[j][kb], th[j][ka]);
       const int zn = (z + 1 == L) ? 0 : z + 1, zq = (z == 0) ? L - 1 : z - 1;
       const int row = L * (y + L * z);
       const float2 nxp = xbuf[(xp + row) >> 1];
@@ -219,11 +229,12 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
       const bool acc = (de <= 0.0f) | (ua < pacc);
       nacc_wave += __popcll(__ballot(acc));  // scalar count per wave
       const bool wa = acc & !z0, wb = acc & (z0 != 0);
-      th[ka] = wa ? t1 : th[ka];
-      th[kb] = wb ? t1 : th[kb];
+      th[j][ka] = wa ? t1 : th[j][ka];
+      th[j][kb] = wb ? t1 : th[j][kb];
       // one site in flight per thread: keeps the register budget of the
       // 1024-thread L=32 workgroup (occupancy supplies the latency hiding)
       __builtin_amdgcn_sched_barrier(0);
+    }
     }
   };
 
@@ -235,13 +246,17 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
     if (own) {
       const int z0 = (x + y) & 1;  // colour-0 sites of the column: z = 2k + z0
       const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+      const int y0 = y;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+      const int y = y0 + j * S::YS;
       const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
 #pragma unroll
       for (int k = 0; k < L / 2; ++k) {
         const int z = 2 * k + z0;
         const int ka = 2 * k, kb = 2 * k + 1;
         const int kn = (2 * k + 2) % L, kp = (2 * k - 1 + L) % L;
-        const float t0 = sel(z0, th[kb], th[ka]);
+        const float t0 = sel(z0, th[j][kb], th[j][ka]);
         const float cc = cos_t(t0), ss = sin_t(t0);
         const int row = L * (y + L * z);
         const float2 nxp = xbuf[(xp + row) >> 1];
@@ -263,6 +278,7 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
         mx += cc + nzp.x;  // the colour-0 site and its colour-1 z+ neighbour
         my += ss + nzp.y;
         __builtin_amdgcn_sched_barrier(0);
+      }
       }
     }
     double v[9] = {-(double)(e_c[0] + e_c[1] + e_c[2]), (double)mx, (double)my,
@@ -298,23 +314,29 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
     // r_i = cos(theta_i - phi) (src/updates.rs:579-582)
     if (own) {
 #pragma unroll
-      for (int z = 0; z < L; ++z) rbuf[x + L * (y + L * z)] = cos_t(th[z] - phit);
+      for (int j = 0; j < CPT; ++j)
+#pragma unroll
+        for (int z = 0; z < L; ++z)
+          rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(th[j][z] - phit);
     }
     __syncthreads();
     const int wave = t >> 6, lane = t & 63;
     const float m2b = -2.0f * beta_l2e;
-    const int xp = (x + 1 == L) ? 0 : x + 1, yp = (y + 1 == L) ? 0 : y + 1;
+    const int xp = (x + 1 == L) ? 0 : x + 1;
     // bond decisions read r from LDS only, so this loop stays rolled
 #pragma unroll 1
-    for (int z = 0; z < L; ++z) {
+    for (int jz = 0; jz < CPT * L; ++jz) {
+      const int j = jz / L, z = jz % L;
+      const int yj = y + j * S::YS;
+      const int yp = (yj + 1 == L) ? 0 : yj + 1;
       const int zn = (z + 1 == L) ? 0 : z + 1;
       bool bx_ = false, by_ = false, bz_ = false;
       if (own) {
-        const int i = x + L * (y + L * z);
+        const int i = x + L * (yj + L * z);
         const float ri = rbuf[i];
-        const float rxp = rbuf[xp + L * (y + L * z)];
+        const float rxp = rbuf[xp + L * (yj + L * z)];
         const float ryp = rbuf[x + L * (yp + L * z)];
-        const float rzp = rbuf[x + L * (y + L * zn)];
+        const float rzp = rbuf[x + L * (yj + L * zn)];
         const float4 jj = jt[z];
         const Words w = keyed(k0, k1, (uint32_t)i, wctr, TAG_WBOND);
         const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
@@ -324,7 +346,7 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
       }
       const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
       if (lane < S::RPW) {
-        const int row = wave * S::RPW + lane + L * z;
+        const int row = wave * S::RPW + lane + j * S::YS + L * z;
         const int sh = lane * S::RW;
         bxm[row] = (uint32_t)(gx >> sh) & S::MASK;
         bym[row] = (uint32_t)(gy >> sh) & S::MASK;
@@ -364,10 +386,12 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
     if (own) {
       const float refl = 2.0f * phit + 0.5f;
 #pragma unroll
-      for (int z = 0; z < L; ++z) {
-        const bool in = (vis[y + L * z] >> x) & 1u;
-        th[z] = in ? fract_hw(refl - th[z]) : th[z];
-      }
+      for (int j = 0; j < CPT; ++j)
+#pragma unroll
+        for (int z = 0; z < L; ++z) {
+          const bool in = (vis[y + j * S::YS + L * z] >> x) & 1u;
+          th[j][z] = in ? fract_hw(refl - th[j][z]) : th[j][z];
+        }
     }
     int members = 0;
     for (int r = t; r < S::ROWS; r += S::NT) members += __popc(vis[r]);
@@ -400,12 +424,14 @@ __global__ void __launch_bounds__(RShape<L>::NT) k_resident(ResArgs a) {
   // write back the angles (turns) and their trig cache, counters, tallies
   if (own) {
 #pragma unroll
-    for (int z = 0; z < L; ++z) {
-      const int64_t g = base + x + L * (y + L * z);
-      a.theta[g] = th[z];
-      a.cth[g] = cos_t(th[z]);
-      a.sth[g] = sin_t(th[z]);
-    }
+    for (int j = 0; j < CPT; ++j)
+#pragma unroll
+      for (int z = 0; z < L; ++z) {
+        const int64_t g = base + x + L * (y + j * S::YS + L * z);
+        a.theta[g] = th[j][z];
+        a.cth[g] = cos_t(th[j][z]);
+        a.sth[g] = sin_t(th[j][z]);
+      }
   }
   double tallies[2] = {(t & 63) == 0 ? (double)nacc_wave : 0.0, (double)ncluster};
   __syncthreads();
@@ -427,6 +453,7 @@ __global__ void k_zero_tallies(unsigned long long* acc, long long* cs, int n) {
 }
 
 bool resident_supports(int L) { return L == 8 || L == 12 || L == 16 || L == 24 || L == 32; }
+static int default_cpt(int L);
 
 int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan) {
   plan.groups.clear();
@@ -449,80 +476,244 @@ int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan) {
                   hipMemcpyHostToDevice) != hipSuccess)
       return fail(SLSF_E_DEVICE, "hipMemcpy failed for the resident plan");
     g.d_index = (int32_t*)p;
+    g.cpt = default_cpt(g.L);
     plan.groups.push_back(std::move(g));
   }
   return SLSF_OK;
 }
 
 static int default_sweeps_per_launch(int L) { return L >= 24 ? 50 : 100; }
+static int default_cpt(int L) { (void)L; return 1; }
+
+template <int L, int CPT>
+static void launch_resident(const ResArgs& a, int nblocks, bool paired, hipStream_t s) {
+  if (paired) k_resident<L, CPT, true><<<nblocks, RShape<L, CPT>::NT, 0, s>>>(a);
+  else k_resident<L, CPT, false><<<nblocks, RShape<L, CPT>::NT, 0, s>>>(a);
+}
 
 template <int L>
-static void launch_resident(const ResArgs& a, int nblocks, bool paired, hipStream_t s) {
-  if (paired) k_resident<L, true><<<nblocks, RShape<L>::NT, 0, s>>>(a);
-  else k_resident<L, false><<<nblocks, RShape<L>::NT, 0, s>>>(a);
+static void launch_cpt(int cpt, const ResArgs& a, int nblocks, bool paired, hipStream_t s) {
+  if (cpt == 2) launch_resident<L, 2>(a, nblocks, paired, s);
+  else launch_resident<L, 1>(a, nblocks, paired, s);
 }
 
-static void launch_any(int L, const ResArgs& a, int nblocks, bool paired, hipStream_t s) {
+static void launch_any(int L, int cpt, const ResArgs& a, int nblocks, bool paired,
+                       hipStream_t s) {
   switch (L) {
-    case 8: launch_resident<8>(a, nblocks, paired, s); break;
-    case 12: launch_resident<12>(a, nblocks, paired, s); break;
-    case 16: launch_resident<16>(a, nblocks, paired, s); break;
-    case 24: launch_resident<24>(a, nblocks, paired, s); break;
-    default: launch_resident<32>(a, nblocks, paired, s); break;
+    case 8: launch_resident<8, 1>(a, nblocks, paired, s); break;  // L/2 = 4: one column
+    case 12: launch_resident<12, 1>(a, nblocks, paired, s); break;
+    case 16: launch_cpt<16>(cpt, a, nblocks, paired, s); break;
+    case 24: launch_cpt<24>(cpt, a, nblocks, paired, s); break;
+    default: launch_cpt<32>(cpt, a, nblocks, paired, s); break;
   }
 }
+
+std::vector<int> resident_cpt_candidates(int L) {
+  if (L == 16 || L == 24 || L == 32) return {1, 2};
+  return {1};
+}
+
+int resident_waves_per_replica(int L, int cpt) {
+  const int rw = L <= 8 ? 8 : (L <= 16 ? 16 : 32);
+  return rw * (L / cpt) / 64;
+}
+
+static ResArgs make_args(const ResidentGroup& grp, const GlobalState& g,
+                         const ResidentRunArgs& ra) {
+  ResArgs a;
+  a.reps = g.reps;
+  a.index = grp.d_index;
+  a.theta = (float*)g.theta;
+  a.cth = (float*)g.cth;
+  a.sth = (float*)g.sth;
+  a.jxy = (const float*)g.jxy;
+  a.jz = (const float*)g.jz;
+  a.accum = ra.accum;
+  a.bins = ra.bins;
+  a.bin_state = ra.bin_state;
+  a.acc_out = ra.acc_out;
+  a.csize_out = ra.csize_out;
+  a.bin_cap = ra.bin_cap;
+  a.binsize = ra.binsize;
+  a.nsweeps = 0;
+  a.measure = 0;
+  return a;
+}
+
+// Enqueues every launch of one group for the run on stream s; returns the count.
+static int64_t enqueue_group(const ResidentGroup& grp, const GlobalState& g,
+                             const ResidentRunArgs& ra, hipStream_t s) {
+  ResArgs a = make_args(grp, g, ra);
+  const int spl = ra.sweeps_per_launch > 0   ? ra.sweeps_per_launch
+                  : grp.sweeps_per_launch > 0 ? grp.sweeps_per_launch
+                                              : default_sweeps_per_launch(grp.L);
+  const int nb = (int)grp.replicas.size();
+  int64_t launches = 0;
+  for (int phase = 0; phase < 2; ++phase) {
+    int64_t left = phase == 0 ? ra.thermalization : ra.sweeps;
+    a.measure = phase;
+    while (left > 0) {
+      a.nsweeps = (int)std::min<int64_t>(left, spl);
+      launch_any(grp.L, grp.cpt, a, nb, !ra.replay_stream, s);
+      ++launches;
+      left -= a.nsweeps;
+    }
+  }
+  return launches;
+}
+
+#define RES_TRY(expr, msg)                                    \
+  do {                                                        \
+    if ((expr) != hipSuccess) return fail(SLSF_E_DEVICE, msg); \
+  } while (0)
 
 int resident_run(ResidentPlan& plan, const GlobalState& g, const ResidentRunArgs& ra,
                  hipStream_t s, ResidentRunResult* out) {
-  k_zero_tallies<<<(g.nrep + 255) / 256, 256, 0, s>>>(ra.acc_out, ra.csize_out, g.nrep);
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-    return fail(SLSF_E_DEVICE, "hipEventCreate failed");
-  double kernel_ms = 0.0;
-  int64_t launches = 0;
-  for (const ResidentGroup& grp : plan.groups) {
-    ResArgs a;
-    a.reps = g.reps;
-    a.index = grp.d_index;
-    a.theta = (float*)g.theta;
-    a.cth = (float*)g.cth;
-    a.sth = (float*)g.sth;
-    a.jxy = (const float*)g.jxy;
-    a.jz = (const float*)g.jz;
-    a.accum = ra.accum;
-    a.bins = ra.bins;
-    a.bin_state = ra.bin_state;
-    a.acc_out = ra.acc_out;
-    a.csize_out = ra.csize_out;
-    a.bin_cap = ra.bin_cap;
-    a.binsize = ra.binsize;
-    const int spl = ra.sweeps_per_launch > 0 ? ra.sweeps_per_launch
-                                             : default_sweeps_per_launch(grp.L);
-    const int nb = (int)grp.replicas.size();
-    for (int phase = 0; phase < 2; ++phase) {
-      int64_t left = phase == 0 ? ra.thermalization : ra.sweeps;
-      a.measure = phase;
-      while (left > 0) {
-        a.nsweeps = (int)std::min<int64_t>(left, spl);
-        hipEventRecord(e0, s);
-        launch_any(grp.L, a, nb, !ra.replay_stream, s);
-        hipEventRecord(e1, s);
-        if (hipGetLastError() != hipSuccess) return fail(SLSF_E_DEVICE, "resident launch failed");
-        if (hipEventSynchronize(e1) != hipSuccess)
-          return fail(SLSF_E_DEVICE, "resident kernel failed");
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        kernel_ms += ms;
-        ++launches;
-        left -= a.nsweeps;
-      }
-    }
+  const size_t ng = plan.groups.size();
+  // one stream per extra group so uneven sizes share the GPU (default priority)
+  while (plan.streams.size() + 1 < ng) {
+    hipStream_t t;
+    RES_TRY(hipStreamCreateWithFlags(&t, hipStreamNonBlocking), "hipStreamCreate failed");
+    plan.streams.push_back(t);
+    hipEvent_t e;
+    RES_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate failed");
+    plan.events.push_back(e);
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  out->kernel_ms = kernel_ms;
+  if (!plan.fork) {
+    RES_TRY(hipEventCreateWithFlags(&plan.fork, hipEventDisableTiming), "hipEventCreate failed");
+    RES_TRY(hipEventCreate(&plan.t0), "hipEventCreate failed");
+    RES_TRY(hipEventCreate(&plan.t1), "hipEventCreate failed");
+  }
+  k_zero_tallies<<<(g.nrep + 255) / 256, 256, 0, s>>>(ra.acc_out, ra.csize_out, g.nrep);
+  RES_TRY(hipEventRecord(plan.t0, s), "hipEventRecord failed");
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (ra.use_graph)
+    RES_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "graph capture failed");
+  int64_t launches = 0;
+  if (ng > 1) RES_TRY(hipEventRecord(plan.fork, s), "hipEventRecord failed");
+  for (size_t i = 0; i < ng; ++i) {
+    hipStream_t gs = i == 0 ? s : plan.streams[i - 1];
+    if (i > 0) RES_TRY(hipStreamWaitEvent(gs, plan.fork, 0), "hipStreamWaitEvent failed");
+    launches += enqueue_group(plan.groups[i], g, ra, gs);
+  }
+  for (size_t i = 1; i < ng; ++i) {
+    RES_TRY(hipEventRecord(plan.events[i - 1], plan.streams[i - 1]), "hipEventRecord failed");
+    RES_TRY(hipStreamWaitEvent(s, plan.events[i - 1], 0), "hipStreamWaitEvent failed");
+  }
+  if (ra.use_graph) {
+    RES_TRY(hipStreamEndCapture(s, &graph), "graph capture failed");
+    RES_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate failed");
+    RES_TRY(hipGraphLaunch(exec, s), "graph launch failed");
+  }
+  RES_TRY(hipEventRecord(plan.t1, s), "hipEventRecord failed");
+  if (hipGetLastError() != hipSuccess) return fail(SLSF_E_DEVICE, "resident launch failed");
+  const hipError_t e = hipEventSynchronize(plan.t1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(SLSF_E_DEVICE, std::string("resident kernel failed: ") +
+                                                      hipGetErrorString(e));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, plan.t0, plan.t1);
+  out->kernel_ms = ms;
   out->launches = launches;
   return SLSF_OK;
+}
+
+void resident_release(ResidentPlan& plan) {
+  for (auto& grp : plan.groups)
+    if (grp.d_index) (void)hipFree(grp.d_index);
+  for (hipStream_t t : plan.streams) (void)hipStreamDestroy(t);
+  for (hipEvent_t e : plan.events) (void)hipEventDestroy(e);
+  if (plan.fork) (void)hipEventDestroy(plan.fork);
+  if (plan.t0) (void)hipEventDestroy(plan.t0);
+  if (plan.t1) (void)hipEventDestroy(plan.t1);
+  plan = ResidentPlan();
+}
+
+// Times every (columns per thread, sweeps per launch) candidate of each group
+// on the batch's own replicas without measurement, then restores the replicas'
+// angles, trig caches and counters, so tuning leaves no trace in the chains.
+int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
+                      int trial_sweeps, std::vector<TuneRecord>& records) {
+  records.clear();
+  const size_t n = (size_t)g.total_sites;
+  float *th = nullptr, *c = nullptr, *sn = nullptr;
+  RepDev* rd = nullptr;
+  RES_TRY(hipMalloc((void**)&th, n * sizeof(float)), "hipMalloc failed");
+  RES_TRY(hipMalloc((void**)&c, n * sizeof(float)), "hipMalloc failed");
+  RES_TRY(hipMalloc((void**)&sn, n * sizeof(float)), "hipMalloc failed");
+  RES_TRY(hipMalloc((void**)&rd, (size_t)g.nrep * sizeof(RepDev)), "hipMalloc failed");
+  auto copy_all = [&](bool save) -> hipError_t {
+    hipError_t e = hipSuccess;
+    auto cp = [&](void* dst, const void* src, size_t b) {
+      if (e == hipSuccess) e = hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s);
+    };
+    if (save) {
+      cp(th, g.theta, n * 4); cp(c, g.cth, n * 4); cp(sn, g.sth, n * 4);
+      cp(rd, g.reps, g.nrep * sizeof(RepDev));
+    } else {
+      cp(g.theta, th, n * 4); cp(g.cth, c, n * 4); cp(g.sth, sn, n * 4);
+      cp(g.reps, rd, g.nrep * sizeof(RepDev));
+    }
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+  };
+  int rc = copy_all(true) == hipSuccess ? SLSF_OK : fail(SLSF_E_DEVICE, "autotune save failed");
+  unsigned long long* acc = nullptr;
+  long long* cs = nullptr;
+  if (!rc && (hipMalloc((void**)&acc, g.nrep * 8) != hipSuccess ||
+              hipMalloc((void**)&cs, g.nrep * 8) != hipSuccess))
+    rc = fail(SLSF_E_DEVICE, "hipMalloc failed");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+  }
+  for (size_t gi = 0; !rc && gi < plan.groups.size(); ++gi) {
+    ResidentGroup& grp = plan.groups[gi];
+    TuneRecord best;
+    best.L = grp.L;
+    best.rate = -1.0;
+    for (int cpt : resident_cpt_candidates(grp.L))
+      for (int spl : {16, 64}) {
+        ResidentGroup trial = grp;
+        trial.cpt = cpt;
+        trial.sweeps_per_launch = spl;
+        ResidentRunArgs ra;
+        ra.thermalization = std::max(trial_sweeps, spl);
+        ra.acc_out = acc;
+        ra.csize_out = cs;
+        // warm the code object, then time
+        ResidentRunArgs warm = ra;
+        warm.thermalization = 1;
+        enqueue_group(trial, g, warm, s);
+        (void)hipEventRecord(e0, s);
+        enqueue_group(trial, g, ra, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = fail(SLSF_E_DEVICE, "autotune trial failed");
+          break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double rate = (double)grp.replicas.size() * grp.L * grp.L * grp.L *
+                            (double)ra.thermalization / (ms * 1e-3);
+        TuneRecord rec{grp.L, cpt, spl, resident_waves_per_replica(grp.L, cpt), rate};
+        records.push_back(rec);
+        if (rate > best.rate) best = rec;
+      }
+    if (!rc) {
+      grp.cpt = best.cpt;
+      grp.sweeps_per_launch = best.spl;
+    }
+  }
+  if (!rc && copy_all(false) != hipSuccess) rc = fail(SLSF_E_DEVICE, "autotune restore failed");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(th); (void)hipFree(c); (void)hipFree(sn); (void)hipFree(rd);
+  if (acc) (void)hipFree(acc);
+  if (cs) (void)hipFree(cs);
+  return rc;
 }
 
 }  // namespace slsf

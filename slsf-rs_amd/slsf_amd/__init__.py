@@ -70,6 +70,13 @@ class RunStats(ctypes.Structure):
                 ("mean_cluster_size", ctypes.c_double)]
 
 
+class TuneRecord(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_int32), ("columns_per_thread", ctypes.c_int32),
+                ("waves_per_replica", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
+                ("site_updates_per_s", ctypes.c_double), ("chosen", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class Task(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 128),
                 ("l", ctypes.c_int32), ("l_x", ctypes.c_int32), ("l_y", ctypes.c_int32),
@@ -138,6 +145,8 @@ _SIGS = {
     "slsf_get_bins": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _dp,
                                      ctypes.c_int32, _i32p]),
     "slsf_get_acceptance": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "slsf_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _P(TuneRecord),
+                                     ctypes.c_int32, _i32p]),
     "slsf_binned_estimate": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int32, _dp, _dp,
                                             _dp, _i64p]),
     "slsf_blocking_stderr": (ctypes.c_double, [_dp, ctypes.c_int64]),
@@ -338,6 +347,15 @@ class Batch:
         rows = [self.bins(replica, s) for s in range(len(SERIES))]
         n = min(len(r) for r in rows)
         return np.stack([r[:n] for r in rows])
+
+    def autotune(self, trial_sweeps: int = 16) -> list[dict]:
+        """Pick waves per replica and sweeps per launch for every replica size;
+        the chains are left exactly as they were."""
+        recs = (TuneRecord * 64)()
+        n = ctypes.c_int32()
+        _check(_lib.slsf_autotune(self._h, trial_sweeps, recs, 64, ctypes.byref(n)))
+        return [{k: getattr(r, k) for k, _ in TuneRecord._fields_ if k != "reserved"}
+                for r in recs[:n.value]]
 
     def acceptance(self) -> np.ndarray:
         out = np.zeros(len(self))

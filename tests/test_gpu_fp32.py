@@ -213,3 +213,46 @@ def test_checkpoint_resume_equivalence(precision):
     c.run(18, 0, 1)
     assert np.array_equal(a.get_theta(0), c.get_theta(0))
     assert list(a.counters()[0]) == list(c.counters()[0]) == [30]
+
+
+def _uneven_reps():
+    reps = []
+    for i, L in enumerate([8, 12, 16, 24, 32]):
+        reps += reps_for(L, 3, T=1.1 + 0.1 * i, seed0=77 * L)
+    return reps
+
+
+def test_autotune_picks_and_leaves_chains_unchanged():
+    """Tuning times candidates on the batch's own replicas and restores them:
+    angles after tune + run equal those after run alone (waves per replica and
+    sweeps per launch do not change any site's arithmetic)."""
+    reps = _uneven_reps()
+    a = slsf.Batch(reps, slsf.FP32)
+    b = slsf.Batch(reps, slsf.FP32)
+    a.init_angles()
+    b.init_angles()
+    recs = a.autotune(trial_sweeps=8)
+    assert {r["l"] for r in recs} == {8, 12, 16, 24, 32}
+    for L in (8, 12, 16, 24, 32):
+        chosen = [r for r in recs if r["l"] == L and r["chosen"]]
+        assert len(chosen) >= 1 and all(r["site_updates_per_s"] > 0 for r in recs)
+    assert list(a.counters()[0]) == [0] * len(reps)
+    a.run(10, 10, 5)
+    b.run(10, 10, 5)
+    for i in range(len(reps)):
+        assert np.array_equal(a.get_theta(i), b.get_theta(i)), i
+        assert np.allclose(a.all_bins(i), b.all_bins(i), rtol=2e-5, atol=1e-6)
+
+
+def test_graph_capture_matches_stream_launches():
+    reps = _uneven_reps()
+    out = []
+    for graph in (False, True):
+        b = slsf.Batch(reps, slsf.FP32)
+        b.init_angles()
+        st = b.run(12, 24, 6, sweeps_per_launch=5, use_graph=graph)
+        assert st["launches"] == 5 * (3 + 5)
+        out.append((np.concatenate([b.get_theta(i) for i in range(len(reps))]),
+                    np.stack([b.all_bins(i) for i in range(len(reps))])))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
