@@ -190,7 +190,9 @@ def _oracle_driver(dims, T, jxy, jz, seed, **opt):
                         opt.get("wolff_steps", 1), opt.get("correlation_interval", 1),
                         opt.get("correlation_rmax", -1), -1, -1, int(opt.get("cold_init", 0)), 0)
     res = orc.OSimResult()
-    bufs = [np.zeros(64) for _ in range(5)]
+    # capacities the driver requires: l_x/2, l_z * l_x/2, l_z/2 (generously sized)
+    cap = dims[0] * dims[2] + 64
+    bufs = [np.zeros(cap) for _ in range(5)]
     res.corr_x, res.corr_y, res.corr_xy, res.corr_xy_by_z, res.corr_z = (
         b.ctypes.data_as(orc._dp) for b in bufs)
     import ctypes

@@ -457,7 +457,7 @@ def _opts(**kw):
 def _run_sim(dims, params, opts, r):
     lat = orc.Lattice(*dims)
     res = orc.OSimResult()
-    bufs = [np.zeros(64) for _ in range(5)]
+    bufs = [np.zeros(dims[0] * dims[2] + 64) for _ in range(5)]
     res.corr_x, res.corr_y, res.corr_xy, res.corr_xy_by_z, res.corr_z = (
         b.ctypes.data_as(orc._dp) for b in bufs)
     rc = L.or_run_theta_simulation(lat.ref, *params, ctypes.byref(opts), ctypes.byref(r),
