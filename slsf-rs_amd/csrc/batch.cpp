@@ -295,6 +295,10 @@ int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t prec
     d.key1 = (uint32_t)(r.seed >> 32);
     d.beta = 1.0 / r.temperature;  // src/updates.rs:453
     d.width = r.width;
+    d.measure_from = UINT32_MAX;  // outside slsf_run nothing is binned
+    d.stop_at = UINT32_MAX;       // and nothing stops a replica
+    d.active = 1;
+    d.binsize = 1;
   }
   if ((rc = push_reps_to_device(bp))) return rc;
   rc = precision == SLSF_FP64_REPLAY ? upload_couplings<double>(bp) : upload_couplings<float>(bp);
@@ -574,7 +578,7 @@ int slsf_get_acceptance(slsf_batch* b, double* acceptance) {
 // ---------------------------------------------------------------------------
 namespace slsf {
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t binsize, int32_t bin_cap, hipStream_t s);
+                       int32_t bin_cap, hipStream_t s);
 }
 
 namespace {
@@ -638,6 +642,23 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
   if (opt->stream != SLSF_STREAM_PRODUCTION && opt->stream != SLSF_STREAM_REPLAY)
     return fail(SLSF_E_INVALID, "stream must be SLSF_STREAM_PRODUCTION or SLSF_STREAM_REPLAY");
   if (int rc = ensure_bins(b, opt)) return rc;
+  // run window of every replica: measured from counter + thermalization, stop
+  // after thermalization + sweeps (kernels enforce both per replica)
+  if (int rc = sync_reps_from_device(b)) return rc;
+  for (auto& d : b->reps_host) {
+    d.measure_from = d.metro_ctr + (uint32_t)opt->thermalization;
+    d.stop_at = d.metro_ctr + (uint32_t)(opt->thermalization + opt->sweeps);
+    d.binsize = opt->binsize > 0 ? opt->binsize : 1;
+  }
+  if (int rc = push_reps_to_device(b)) return rc;
+  struct RestoreWindow {
+    slsf_batch* b;
+    ~RestoreWindow() {
+      if (sync_reps_from_device(b)) return;
+      for (auto& d : b->reps_host) { d.measure_from = UINT32_MAX; d.stop_at = UINT32_MAX; }
+      (void)push_reps_to_device(b);
+    }
+  } restore{b};
   const auto t0 = std::chrono::steady_clock::now();
   int64_t launches = 0;
   double kernel_ms = 0.0;
@@ -687,8 +708,7 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
       if (s >= opt->thermalization) {
         if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
         else launch_observables<float>(b->g, b->stream);
-        launch_accumulate(b->g, b->d_accum, b->d_bins, b->d_bin_state, b->binsize, b->bin_cap,
-                          b->stream);
+        launch_accumulate(b->g, b->d_accum, b->d_bins, b->d_bin_state, b->bin_cap, b->stream);
         launches += 2;
         HIP_TRY(hipGetLastError());
       }

@@ -21,13 +21,17 @@ namespace slsf {
 
 __global__ void k_accumulate(const RepDev* __restrict__ reps, int nrep,
                              const double* __restrict__ partials, int obs_blocks, double* accum,
-                             double* bins, int32_t* bin_state, int32_t binsize, int32_t bin_cap) {
+                             double* bins, int32_t* bin_state, int32_t bin_cap) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrep) return;
+  const RepDev d = reps[r];
+  // measured iff this sweep took part and its index metro_ctr - 1 >= measure_from
+  if (!d.active || d.metro_ctr == 0u || d.metro_ctr - 1u < d.measure_from) return;
+  const int32_t binsize = d.binsize;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = 0; b < obs_blocks; ++b)
     for (int k = 0; k < 9; ++k) v[k] += partials[((int64_t)r * obs_blocks + b) * 9 + k];
-  const double vol = (double)reps[r].volume;
+  const double vol = (double)d.volume;
   const double e = v[0] / vol;
   const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
   const double vals[13] = {e,    e * e, m2,   m2 * m2,     v[3],        v[4],       v[5],
@@ -47,9 +51,9 @@ __global__ void k_accumulate(const RepDev* __restrict__ reps, int nrep,
 }
 
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t binsize, int32_t bin_cap, hipStream_t s) {
+                       int32_t bin_cap, hipStream_t s) {
   k_accumulate<<<(g.nrep + 63) / 64, 64, 0, s>>>(g.reps, g.nrep, g.partials, g.obs_blocks,
-                                                  accum, bins, bin_state, binsize, bin_cap);
+                                                  accum, bins, bin_state, bin_cap);
 }
 
 }  // namespace slsf

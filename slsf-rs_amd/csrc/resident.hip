@@ -71,7 +71,7 @@ struct ResArgs {
   int32_t* bin_state;
   unsigned long long* acc_out;
   long long* csize_out;
-  int32_t bin_cap, binsize, nsweeps, measure;
+  int32_t bin_cap, nsweeps;
 };
 
 template <int L>
@@ -295,10 +295,10 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       for (int k = 0; k < 13; ++k) acc[k] += vals[k];
       int32_t* st = a.bin_state + 2 * rep;
       st[0] += 1;
-      if (st[0] == a.binsize) {
+      if (st[0] == d.binsize) {
         if (st[1] < a.bin_cap)
           for (int k = 0; k < 13; ++k)
-            a.bins[((int64_t)rep * 13 + k) * a.bin_cap + st[1]] = acc[k] / (double)a.binsize;
+            a.bins[((int64_t)rep * 13 + k) * a.bin_cap + st[1]] = acc[k] / (double)d.binsize;
         for (int k = 0; k < 13; ++k) acc[k] = 0.0;
         st[0] = 0;
         st[1] += 1;
@@ -399,23 +399,28 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     wctr += 1u;
   };
 
-  for (int sw = 0; sw < a.nsweeps; ++sw) {
+  // the replica's own window decides how many sweeps it does and which are
+  // measured (uniform per workgroup); a measurement is taken at the next
+  // sweep's first exchange, or after the loop for the last sweep
+  bool pending = false;
+  for (int sw = 0; sw < a.nsweeps && mctr < d.stop_at; ++sw) {
     exchange(1);
     __syncthreads();
-    if (a.measure && sw > 0) measure();
+    if (pending) measure();
     update(0);
     __syncthreads();
     exchange(0);
     __syncthreads();
     update(1);
     __syncthreads();
+    pending = mctr >= d.measure_from;
     mctr += 1u;
     for (int k = 0; k < d.wolff_steps; ++k) {
       wolff();
       __syncthreads();
     }
   }
-  if (a.measure && a.nsweeps > 0) {
+  if (pending) {
     exchange(1);
     __syncthreads();
     measure();
@@ -534,9 +539,7 @@ static ResArgs make_args(const ResidentGroup& grp, const GlobalState& g,
   a.acc_out = ra.acc_out;
   a.csize_out = ra.csize_out;
   a.bin_cap = ra.bin_cap;
-  a.binsize = ra.binsize;
   a.nsweeps = 0;
-  a.measure = 0;
   return a;
 }
 
@@ -549,15 +552,12 @@ static int64_t enqueue_group(const ResidentGroup& grp, const GlobalState& g,
                                               : default_sweeps_per_launch(grp.L);
   const int nb = (int)grp.replicas.size();
   int64_t launches = 0;
-  for (int phase = 0; phase < 2; ++phase) {
-    int64_t left = phase == 0 ? ra.thermalization : ra.sweeps;
-    a.measure = phase;
-    while (left > 0) {
-      a.nsweeps = (int)std::min<int64_t>(left, spl);
-      launch_any(grp.L, grp.cpt, a, nb, !ra.replay_stream, s);
-      ++launches;
-      left -= a.nsweeps;
-    }
+  int64_t left = ra.thermalization + ra.sweeps;  // the replicas' windows cut the tail
+  while (left > 0) {
+    a.nsweeps = (int)std::min<int64_t>(left, spl);
+    launch_any(grp.L, grp.cpt, a, nb, !ra.replay_stream, s);
+    ++launches;
+    left -= a.nsweeps;
   }
   return launches;
 }

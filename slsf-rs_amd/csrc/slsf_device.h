@@ -176,9 +176,14 @@ struct RepDev {
   uint32_t key0, key1;
   uint32_t metro_ctr;  // Metropolis sweeps done (counter of TAG_METRO)
   uint32_t wolff_ctr;  // Wolff clusters done (counter of TAG_WSEED/TAG_WBOND)
-  int32_t pad;
+  uint32_t measure_from;  // sweeps with counter >= this are measured
+  uint32_t stop_at;       // the replica does no sweep once metro_ctr reaches this
+  uint32_t active;        // set per sweep by the HBM path: metro_ctr < stop_at
+  int32_t binsize;        // measurements per bin
+  int32_t pad[2];
   double beta, width;
 };
+static_assert(sizeof(RepDev) == 96, "RepDev layout");
 
 struct WolffDev {  // per replica, per cluster step
   double phi, sphi, cphi;
