@@ -291,6 +291,65 @@ int slsf_jackknife(const double* loo, int64_t nbins, double f_all, double* estim
                    double* stderr_out);
 
 /* ======================================================================== */
+/* Serving: continuous batching of independent jobs                          */
+/* ======================================================================== */
+/* No reference counterpart: the reference runs each task in its own process
+ * (src/results.rs:187-239).  A server keeps one slot pool per lattice size on
+ * the device; a job occupies a slot from admission to completion, its state
+ * and FP64 accumulators stay resident between chunks, and every step runs one
+ * chunk of sweeps for all active slots of every pool (pools on their own
+ * streams, so their work overlaps).  Jobs arriving while others run are
+ * admitted as soon as a slot of their size is free and the pool fits the
+ * device-memory budget.  A job's result does not depend on what else runs:
+ * its uniforms are keyed by its own seed and counters. */
+typedef struct {
+  int32_t l;                 /* cubic edge, even */
+  int32_t wolff_steps;
+  double temperature;
+  double proposal_width;
+  double j_xy_min, j_xy_max; /* J_xy[z] ~ U[j_xy_min, j_xy_max] iid per layer */
+  double j_perp;             /* J_z[z] for every layer */
+  uint64_t disorder_seed;    /* SplitMix64 seed of the layer couplings */
+  uint64_t seed;             /* key of the job's uniforms */
+  int64_t thermalization, sweeps;
+  int32_t binsize;           /* raised if sweeps/binsize exceeds the pool's bin capacity */
+  int32_t reserved;
+} slsf_job_request;
+
+typedef struct {
+  int64_t job_id;
+  int32_t l, nbins;
+  double submit_s, start_s, finish_s; /* server clock (seconds since create) */
+  double acceptance;
+  double value[SLSF_NUM_EVALS], error[SLSF_NUM_EVALS]; /* slsf_evaluate of the bins */
+  double series_mean[SLSF_NUM_SERIES];
+  int64_t site_updates;
+} slsf_job_result;
+
+typedef struct {
+  int64_t queued, active, finished, device_bytes, site_updates;
+  double clock_s;
+} slsf_server_stats;
+
+typedef struct slsf_server slsf_server;
+
+/* device_budget_bytes: admission limit for all pools; max_slots_per_size:
+ * slots of one lattice size; chunk_sweeps: sweeps per step. */
+int slsf_server_create(int64_t device_budget_bytes, int32_t max_slots_per_size,
+                       int32_t chunk_sweeps, slsf_server** out);
+void slsf_server_destroy(slsf_server* server);
+/* Validates and queues a job (reference messages for bad L, T, width,
+ * couplings); SLSF_E_ADMISSION if one slot of its size can never fit. */
+int slsf_server_submit(slsf_server* server, const slsf_job_request* job, int64_t* job_id);
+/* One scheduling step: admit, run one chunk everywhere, retire finished jobs. */
+int slsf_server_step(slsf_server* server, int32_t* finished);
+/* Drains finished results (oldest first). */
+int slsf_server_poll(slsf_server* server, slsf_job_result* out, int32_t cap, int32_t* n);
+int slsf_server_get_stats(const slsf_server* server, slsf_server_stats* out);
+/* Bytes one slot of size l occupies (the admission footprint). */
+int64_t slsf_server_slot_bytes(int32_t l);
+
+/* ======================================================================== */
 /* Job configuration (TOML) and task expansion                               */
 /* ======================================================================== */
 

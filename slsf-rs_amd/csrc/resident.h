@@ -54,6 +54,10 @@ int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan);
 int resident_run(ResidentPlan& plan, const GlobalState& g, const ResidentRunArgs& a,
                  hipStream_t s, ResidentRunResult* out);
 void resident_release(ResidentPlan& plan);
+// Enqueues (without waiting) the launches of one group for ra.thermalization +
+// ra.sweeps sweeps; returns the number of launches.
+int64_t resident_enqueue(const ResidentGroup& grp, const GlobalState& g,
+                         const ResidentRunArgs& ra, hipStream_t s);
 int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
                       int trial_sweeps, std::vector<TuneRecord>& records);
 std::vector<int> resident_cpt_candidates(int L);

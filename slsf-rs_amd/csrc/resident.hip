@@ -562,6 +562,11 @@ static int64_t enqueue_group(const ResidentGroup& grp, const GlobalState& g,
   return launches;
 }
 
+int64_t resident_enqueue(const ResidentGroup& grp, const GlobalState& g,
+                         const ResidentRunArgs& ra, hipStream_t s) {
+  return enqueue_group(grp, g, ra, s);
+}
+
 #define RES_TRY(expr, msg)                                    \
   do {                                                        \
     if ((expr) != hipSuccess) return fail(SLSF_E_DEVICE, msg); \

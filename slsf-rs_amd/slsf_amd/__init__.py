@@ -77,6 +77,30 @@ class TuneRecord(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
+class JobRequest(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_int32), ("wolff_steps", ctypes.c_int32),
+                ("temperature", ctypes.c_double), ("proposal_width", ctypes.c_double),
+                ("j_xy_min", ctypes.c_double), ("j_xy_max", ctypes.c_double),
+                ("j_perp", ctypes.c_double), ("disorder_seed", ctypes.c_uint64),
+                ("seed", ctypes.c_uint64), ("thermalization", ctypes.c_int64),
+                ("sweeps", ctypes.c_int64), ("binsize", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("job_id", ctypes.c_int64), ("l", ctypes.c_int32), ("nbins", ctypes.c_int32),
+                ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double),
+                ("finish_s", ctypes.c_double), ("acceptance", ctypes.c_double),
+                ("value", ctypes.c_double * 7), ("error", ctypes.c_double * 7),
+                ("series_mean", ctypes.c_double * 13), ("site_updates", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int64), ("active", ctypes.c_int64),
+                ("finished", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                ("site_updates", ctypes.c_int64), ("clock_s", ctypes.c_double)]
+
+
 class Task(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 128),
                 ("l", ctypes.c_int32), ("l_x", ctypes.c_int32), ("l_y", ctypes.c_int32),
@@ -164,6 +188,14 @@ _SIGS = {
     "slsf_job_selected_tasks": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int32,
                                                  ctypes.c_int32, _i64p, ctypes.c_int64]),
     "slsf_parse_duration": (ctypes.c_int, [ctypes.c_char_p, _i64p]),
+    "slsf_server_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                          _P(ctypes.c_void_p)]),
+    "slsf_server_destroy": (None, [ctypes.c_void_p]),
+    "slsf_server_submit": (ctypes.c_int, [ctypes.c_void_p, _P(JobRequest), _i64p]),
+    "slsf_server_step": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
+    "slsf_server_poll": (ctypes.c_int, [ctypes.c_void_p, _P(JobResult), ctypes.c_int32, _i32p]),
+    "slsf_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ServerStats)]),
+    "slsf_server_slot_bytes": (ctypes.c_int64, [ctypes.c_int32]),
     "slsf_run_theta_simulation": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, ctypes.c_double, _dp, _dp,
@@ -428,6 +460,79 @@ def jackknife(loo, f_all: float):
     _check(_lib.slsf_jackknife(_ptr(a, ctypes.c_double), len(a), f_all, ctypes.byref(est),
                                ctypes.byref(err)))
     return est.value, err.value
+
+
+# ---- serving -----------------------------------------------------------------------
+class Server:
+    """Continuous-batching job server (slsf_server_*)."""
+
+    def __init__(self, device_budget_bytes: int, max_slots_per_size: int = 256,
+                 chunk_sweeps: int = 100):
+        h = ctypes.c_void_p()
+        self._h = None
+        _check(_lib.slsf_server_create(int(device_budget_bytes), max_slots_per_size,
+                                       chunk_sweeps, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _lib.slsf_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def slot_bytes(l: int) -> int:
+        return int(_lib.slsf_server_slot_bytes(l))
+
+    def submit(self, l, temperature, seed, disorder_seed, sweeps, thermalization=0,
+               binsize=10, j_xy_min=0.6, j_xy_max=1.8, j_perp=1.0,
+               proposal_width=float(np.pi), wolff_steps=1) -> int:
+        q = JobRequest(l, wolff_steps, temperature, proposal_width, j_xy_min, j_xy_max, j_perp,
+                       disorder_seed & 0xFFFFFFFFFFFFFFFF, seed & 0xFFFFFFFFFFFFFFFF,
+                       thermalization, sweeps, binsize, 0)
+        jid = ctypes.c_int64()
+        _check(_lib.slsf_server_submit(self._h, ctypes.byref(q), ctypes.byref(jid)))
+        return jid.value
+
+    def step(self) -> int:
+        n = ctypes.c_int32()
+        _check(_lib.slsf_server_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def poll(self) -> list[dict]:
+        out = []
+        buf = (JobResult * 256)()
+        while True:
+            n = ctypes.c_int32()
+            _check(_lib.slsf_server_poll(self._h, buf, 256, ctypes.byref(n)))
+            for r in buf[:n.value]:
+                d = {k: getattr(r, k) for k, _ in JobResult._fields_}
+                d["value"] = dict(zip(EVALS, list(r.value)))
+                d["error"] = dict(zip(EVALS, list(r.error)))
+                d["series_mean"] = dict(zip(SERIES, list(r.series_mean)))
+                out.append(d)
+            if n.value < 256:
+                return out
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib.slsf_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def run_until_idle(self, max_steps: int = 1_000_000) -> list[dict]:
+        res = []
+        for _ in range(max_steps):
+            st = self.stats()
+            if st["queued"] == 0 and st["active"] == 0:
+                break
+            self.step()
+            res += self.poll()
+        return res + self.poll()
 
 
 # ---- job configuration -----------------------------------------------------------

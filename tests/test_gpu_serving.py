@@ -1,0 +1,99 @@
+"""GPU tests of the continuous-batching server (slsf_server_*).
+
+* A job's result is bit-identical alone and in a mix of jobs of other sizes,
+  temperatures and lengths, even with a different chunk size (keyed uniforms,
+  per-replica windows and reductions).
+* Jobs that arrive while others run are admitted as slots free up and overlap
+  on the device; a device-memory budget that fits one slot serialises jobs;
+  a job whose slot can never fit is refused with a clear message.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+slsf = pytest.importorskip("slsf_amd")
+
+GB = 1 << 30
+
+
+def job_args(l, seed, sweeps=300, therm=50, T=1.3, binsize=10):
+    return dict(l=l, temperature=T, seed=seed, disorder_seed=seed * 7 + 1, sweeps=sweeps,
+                thermalization=therm, binsize=binsize)
+
+
+def same_result(a, b):
+    assert a["nbins"] == b["nbins"] and a["acceptance"] == b["acceptance"]
+    for k in a["series_mean"]:
+        assert a["series_mean"][k] == b["series_mean"][k], k
+    for k in a["value"]:
+        va, vb = a["value"][k], b["value"][k]
+        assert (va == vb) or (np.isnan(va) and np.isnan(vb)), k
+
+
+def test_job_result_same_alone_and_in_mix():
+    probe_small = job_args(16, 11, sweeps=400, T=1.2)
+    probe_big = job_args(48, 12, sweeps=120, therm=20, T=1.6)
+    alone = {}
+    for name, args in (("small", probe_small), ("big", probe_big)):
+        srv = slsf.Server(8 * GB, 16, chunk_sweeps=50)
+        srv.submit(**args)
+        res = srv.run_until_idle()
+        assert len(res) == 1
+        alone[name] = res[0]
+    mix = slsf.Server(8 * GB, 16, chunk_sweeps=37)
+    ids = {}
+    others = [job_args(32, 100 + i, sweeps=200 + 50 * i, T=0.5 + 0.3 * i) for i in range(4)]
+    others += [job_args(16, 200 + i, sweeps=150, T=2.0) for i in range(3)]
+    others += [job_args(48, 300, sweeps=60, therm=10)]
+    for i, a in enumerate(others[:4]):
+        mix.submit(**a)
+    ids["small"] = mix.submit(**probe_small)
+    mix.step()
+    for a in others[4:]:
+        mix.submit(**a)
+    ids["big"] = mix.submit(**probe_big)
+    res = {r["job_id"]: r for r in mix.run_until_idle()}
+    assert len(res) == len(others) + 2
+    same_result(alone["small"], res[ids["small"]])
+    same_result(alone["big"], res[ids["big"]])
+    for r in res.values():
+        assert r["nbins"] > 0 and np.isfinite(r["value"]["Energy"]) and r["value"]["Energy"] < 0
+
+
+def test_continuous_admission_and_overlap():
+    srv = slsf.Server(8 * GB, 4, chunk_sweeps=25)
+    first = [srv.submit(**job_args(16, 500 + i, sweeps=400)) for i in range(4)]
+    srv.step()
+    late = [srv.submit(**job_args(16, 600 + i, sweeps=100)) for i in range(3)]
+    other = srv.submit(**job_args(32, 700, sweeps=100))
+    assert srv.stats()["queued"] == 4 and srv.stats()["active"] == 4
+    srv.step()  # admits the L=32 job into its own pool; L=16 has 4 slots, all busy
+    st = srv.stats()
+    assert st["queued"] == 3 and st["active"] == 5
+    res = {r["job_id"]: r for r in srv.run_until_idle()}
+    assert len(res) == 8
+    # the L=32 job ran while the first L=16 jobs were running
+    assert res[other]["start_s"] < min(res[j]["finish_s"] for j in first)
+    # late L=16 jobs waited for a free slot, then ran
+    for j in late:
+        assert res[j]["start_s"] >= min(res[f]["finish_s"] for f in first) - 1e-9
+
+
+def test_memory_budget_admission():
+    one = slsf.Server.slot_bytes(16)
+    srv = slsf.Server(one + one // 2, 8, chunk_sweeps=50)  # room for one L=16 slot
+    a = srv.submit(**job_args(16, 1, sweeps=100))
+    b = srv.submit(**job_args(16, 2, sweeps=100))
+    srv.step()
+    assert srv.stats()["active"] == 1 and srv.stats()["queued"] == 1
+    res = {r["job_id"]: r for r in srv.run_until_idle()}
+    assert res[b]["start_s"] >= res[a]["finish_s"] - 1e-9
+    with pytest.raises(slsf.SlsfError) as e:
+        srv.submit(**job_args(64, 3))
+    assert e.value.code == 3 and "more than the device budget" in e.value.message
+    with pytest.raises(slsf.SlsfError) as e:
+        srv.submit(**dict(job_args(16, 4), temperature=0.0))
+    assert e.value.message == "temperature T must be positive and finite"
+    with pytest.raises(slsf.SlsfError) as e:
+        srv.submit(**job_args(15, 4))
+    assert e.value.message == "red-black Metropolis sweep requires even lattice dimensions"
