@@ -149,14 +149,14 @@ int push_reps_to_device(slsf_batch* b) {
 
 bool is_fp64(const slsf_batch* b) { return b->precision == SLSF_FP64_REPLAY; }
 
-int metro_sweep(slsf_batch* b, uint8_t* d_decisions) {
-  launch_clear_counters(b->g, b->stream);
-  if (is_fp64(b)) launch_metro_sweep<double>(b->g, d_decisions, b->stream);
-  else launch_metro_sweep<float>(b->g, d_decisions, b->stream);
+int metro_phases(slsf_batch* b, uint8_t* d_decisions) {
+  if (is_fp64(b)) launch_metro_phases<double>(b->g, d_decisions, b->stream);
+  else launch_metro_phases<float>(b->g, d_decisions, b->stream);
   HIP_TRY(hipGetLastError());
   return SLSF_OK;
 }
 
+// the membership bitmap of this round is copied out when members_host is set
 int wolff_round(slsf_batch* b, int round, uint8_t* members_host) {
   if (is_fp64(b)) launch_wolff<double>(b->g, round, b->stream);
   else launch_wolff<float>(b->g, round, b->stream);
@@ -173,8 +173,15 @@ int wolff_round(slsf_batch* b, int round, uint8_t* members_host) {
         members_host[out] = (uint8_t)((words[gi >> 5] >> (gi & 31)) & 1u);
       }
   }
-  if (is_fp64(b)) launch_wolff_flip_clear<double>(b->g, round, b->stream);
-  else launch_wolff_flip_clear<float>(b->g, round, b->stream);
+  return SLSF_OK;
+}
+
+// one full sweep of the HBM path (Metropolis + the replica's cluster rounds)
+int hbm_sweep(slsf_batch* b, int wmax) {
+  if (int rc = metro_phases(b, nullptr)) return rc;
+  for (int k = 0; k < wmax; ++k)
+    if (int rc = wolff_round(b, k, nullptr)) return rc;
+  launch_sweep_end(b->g, 1, wmax, b->stream);
   HIP_TRY(hipGetLastError());
   return SLSF_OK;
 }
@@ -185,33 +192,51 @@ int max_wolff_steps(const slsf_batch* b) {
   return m;
 }
 
-int gather_acceptance(slsf_batch* b, double* acceptance) {
-  std::vector<unsigned long long> acc(b->reps.size());
-  HIP_TRY(hipMemcpyAsync(acc.data(), b->g.acc, sizeof(unsigned long long) * acc.size(),
+// per-replica sums of the TALLY_SLOTS slots of a device tally
+int read_tallies(slsf_batch* b, const unsigned long long* slots,
+                 std::vector<unsigned long long>& out) {
+  std::vector<unsigned long long> all(b->reps.size() * TALLY_SLOTS);
+  HIP_TRY(hipMemcpyAsync(all.data(), slots, sizeof(unsigned long long) * all.size(),
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  out.assign(b->reps.size(), 0ull);
+  for (size_t r = 0; r < out.size(); ++r)
+    for (int k = 0; k < TALLY_SLOTS; ++k) out[r] += all[r * TALLY_SLOTS + k];
+  return SLSF_OK;
+}
+
+// Reads the device tallies summed since launch_clear_counters over `nsweeps`
+// sweeps: the sum of per-sweep acceptances is accepted / V (every sweep of a
+// replica visits V sites), added to `acceptance` when given.
+int gather_acceptance(slsf_batch* b, int64_t nsweeps, double* acceptance) {
+  std::vector<unsigned long long> acc;
+  if (int rc = read_tallies(b, b->g.acc, acc)) return rc;
   for (size_t r = 0; r < acc.size(); ++r) {
     const double a = (double)acc[r] / (double)b->reps[r].volume;
     b->acc_sum[r] += a;
-    b->acc_count[r] += 1;
+    b->acc_count[r] += nsweeps;
     if (acceptance) acceptance[r] += a;
   }
   return SLSF_OK;
 }
 
-int gather_clusters(slsf_batch* b, int round, int64_t* cluster_size) {
-  std::vector<long long> cs(b->reps.size());
-  HIP_TRY(hipMemcpyAsync(cs.data(), b->g.csize, sizeof(long long) * cs.size(),
-                         hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
+// cluster sites summed since launch_clear_counters; `clusters[r]` is how many
+// clusters replica r grew meanwhile
+int gather_clusters(slsf_batch* b, const std::vector<int64_t>& clusters, int64_t* cluster_size) {
+  std::vector<unsigned long long> cs;
+  if (int rc = read_tallies(b, (unsigned long long*)b->g.csize, cs)) return rc;
   for (size_t r = 0; r < cs.size(); ++r) {
-    if (round < b->reps[r].wolff_steps) {
-      b->cluster_sum[r] += (double)cs[r];
-      b->cluster_count[r] += 1;
-    }
+    b->cluster_sum[r] += (double)cs[r];
+    b->cluster_count[r] += clusters[r];
     if (cluster_size) cluster_size[r] = cs[r];
   }
   return SLSF_OK;
+}
+
+std::vector<int64_t> clusters_in(const slsf_batch* b, int64_t nsweeps) {
+  std::vector<int64_t> n(b->reps.size());
+  for (size_t r = 0; r < n.size(); ++r) n[r] = nsweeps * b->reps[r].wolff_steps;
+  return n;
 }
 
 }  // namespace
@@ -257,31 +282,6 @@ int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t prec
     b->reps.push_back(std::move(r));
   }
   b->total_layers = jz_off;
-  GlobalState& g = b->g;
-  g.nrep = n;
-  g.total_sites = site_off;
-  for (const auto& r : b->reps) g.max_volume = std::max(g.max_volume, r.volume);
-  g.obs_blocks = (int)((g.max_volume + 255) / 256);
-  const size_t rs = precision == SLSF_FP64_REPLAY ? sizeof(double) : sizeof(float);
-  slsf_batch* bp = b.get();
-  int rc = SLSF_OK;
-  rc = rc ? rc : bp->alloc((RepDev**)&g.reps, (size_t)n);
-  rc = rc ? rc : bp->alloc((char**)&g.theta, rs * (size_t)site_off);
-  rc = rc ? rc : bp->alloc((char**)&g.cth, rs * (size_t)site_off);
-  rc = rc ? rc : bp->alloc((char**)&g.sth, rs * (size_t)site_off);
-  rc = rc ? rc : bp->alloc((char**)&g.jxy, rs * (size_t)jz_off);
-  rc = rc ? rc : bp->alloc((char**)&g.jz, rs * (size_t)jz_off);
-  rc = rc ? rc : bp->alloc(&g.bondmask, (size_t)site_off);
-  rc = rc ? rc : bp->alloc(&g.visited, (size_t)site_off / 32);
-  rc = rc ? rc : bp->alloc(&g.qa, (size_t)site_off);
-  rc = rc ? rc : bp->alloc(&g.qb, (size_t)site_off);
-  rc = rc ? rc : bp->alloc(&g.wd, (size_t)n);
-  rc = rc ? rc : bp->alloc(&g.csize, (size_t)n);
-  rc = rc ? rc : bp->alloc(&g.acc, (size_t)n);
-  rc = rc ? rc : bp->alloc(&g.partials, (size_t)n * g.obs_blocks * 9);
-  if (rc) return rc;
-  HIP_TRY(hipMemsetAsync(g.visited, 0, sizeof(uint32_t) * (size_t)(site_off / 32), b->stream));
-  HIP_TRY(hipMemsetAsync(g.theta, 0, rs * (size_t)site_off, b->stream));
   b->reps_host.resize(n);
   for (int i = 0; i < n; ++i) {
     const auto& r = b->reps[i];
@@ -299,7 +299,31 @@ int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t prec
     d.stop_at = UINT32_MAX;       // and nothing stops a replica
     d.active = 1;
     d.binsize = 1;
+    set_divisors(d);
   }
+  GlobalState& g = b->g;
+  g.nrep = n;
+  g.total_sites = site_off;
+  set_extents(g, b->reps_host.data(), n);
+  const size_t rs = precision == SLSF_FP64_REPLAY ? sizeof(double) : sizeof(float);
+  slsf_batch* bp = b.get();
+  int rc = SLSF_OK;
+  rc = rc ? rc : bp->alloc((RepDev**)&g.reps, (size_t)n);
+  rc = rc ? rc : bp->alloc((char**)&g.theta, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.cth, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.sth, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.jxy, rs * (size_t)jz_off);
+  rc = rc ? rc : bp->alloc((char**)&g.jz, rs * (size_t)jz_off);
+  rc = rc ? rc : bp->alloc(&g.bondmask, (size_t)site_off);
+  rc = rc ? rc : bp->alloc(&g.visited, (size_t)site_off / 32);
+  rc = rc ? rc : bp->alloc(&g.label, (size_t)site_off);
+  rc = rc ? rc : bp->alloc(&g.wd, (size_t)n);
+  rc = rc ? rc : bp->alloc(&g.csize, (size_t)n * TALLY_SLOTS);
+  rc = rc ? rc : bp->alloc(&g.acc, (size_t)n * TALLY_SLOTS);
+  rc = rc ? rc : bp->alloc(&g.partials, (size_t)n * g.obs_blocks * 9);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(g.visited, 0, sizeof(uint32_t) * (size_t)(site_off / 32), b->stream));
+  HIP_TRY(hipMemsetAsync(g.theta, 0, rs * (size_t)site_off, b->stream));
   if ((rc = push_reps_to_device(bp))) return rc;
   rc = precision == SLSF_FP64_REPLAY ? upload_couplings<double>(bp) : upload_couplings<float>(bp);
   if (rc) return rc;
@@ -414,8 +438,11 @@ int slsf_metropolis_sweep(slsf_batch* b, double* acceptance, uint8_t* decisions)
   if (decisions && !b->d_decisions)
     if (int rc = b->alloc(&b->d_decisions, (size_t)b->g.total_sites)) return rc;
   if (acceptance) std::fill(acceptance, acceptance + b->reps.size(), 0.0);
-  if (int rc = metro_sweep(b, decisions ? b->d_decisions : nullptr)) return rc;
-  if (int rc = gather_acceptance(b, acceptance)) return rc;
+  launch_clear_counters(b->g, b->stream);
+  launch_sweep_end(b->g, 0, 0, b->stream);
+  if (int rc = metro_phases(b, decisions ? b->d_decisions : nullptr)) return rc;
+  launch_sweep_end(b->g, 1, 0, b->stream);
+  if (int rc = gather_acceptance(b, 1, acceptance)) return rc;
   if (decisions) {
     std::vector<uint8_t> all(b->g.total_sites);
     HIP_TRY(hipMemcpyAsync(all.data(), b->d_decisions, all.size(), hipMemcpyDeviceToHost,
@@ -444,8 +471,13 @@ int slsf_wolff_cluster_step(slsf_batch* b, int64_t* cluster_size, uint8_t* membe
     if (saved[r] < 1) { b->reps_host[r].wolff_steps = 1; patched = true; }
   }
   if (patched) if (int rc = push_reps_to_device(b)) return rc;
+  launch_clear_counters(b->g, b->stream);
+  launch_sweep_end(b->g, 0, 0, b->stream);
   int rc = wolff_round(b, 0, members);
-  if (!rc) rc = gather_clusters(b, 0, cluster_size);
+  if (!rc) {
+    launch_sweep_end(b->g, 0, 1, b->stream);
+    rc = gather_clusters(b, std::vector<int64_t>(b->reps.size(), 1), cluster_size);
+  }
   if (patched) {
     if (int rc2 = sync_reps_from_device(b)) return rc2;
     for (size_t r = 0; r < saved.size(); ++r) b->reps_host[r].wolff_steps = saved[r];
@@ -460,15 +492,12 @@ int slsf_sweep(slsf_batch* b, int32_t nsweeps, double* acceptance_sum) {
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
   if (nsweeps < 0) return fail(SLSF_E_INVALID, "nsweeps must be nonnegative");
   const int wmax = max_wolff_steps(b);
-  for (int s = 0; s < nsweeps; ++s) {
-    if (int rc = metro_sweep(b, nullptr)) return rc;
-    if (int rc = gather_acceptance(b, acceptance_sum)) return rc;
-    for (int k = 0; k < wmax; ++k) {
-      if (int rc = wolff_round(b, k, nullptr)) return rc;
-      if (int rc = gather_clusters(b, k, nullptr)) return rc;
-    }
-  }
-  return SLSF_OK;
+  launch_clear_counters(b->g, b->stream);
+  launch_sweep_end(b->g, 0, 0, b->stream);
+  for (int s = 0; s < nsweeps; ++s)
+    if (int rc = hbm_sweep(b, wmax)) return rc;
+  if (int rc = gather_acceptance(b, nsweeps, acceptance_sum)) return rc;
+  return gather_clusters(b, clusters_in(b, nsweeps), nullptr);
 }
 
 int slsf_measure_observables(slsf_batch* b, slsf_observables* out) {
@@ -477,15 +506,17 @@ int slsf_measure_observables(slsf_batch* b, slsf_observables* out) {
   if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
   else launch_observables<float>(b->g, b->stream);
   HIP_TRY(hipGetLastError());
-  const int nb = b->g.obs_blocks;
-  std::vector<double> part((size_t)b->reps.size() * nb * 9);
+  const int stride = b->g.obs_blocks;
+  std::vector<double> part((size_t)b->reps.size() * stride * 9);
   HIP_TRY(hipMemcpyAsync(part.data(), b->g.partials, sizeof(double) * part.size(),
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   for (size_t r = 0; r < b->reps.size(); ++r) {
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const auto& rr = b->reps[r];
+    const int nb = obs_blocks_of(rr.lx, rr.ly, rr.lz);
     for (int blk = 0; blk < nb; ++blk)
-      for (int k = 0; k < 9; ++k) v[k] += part[((size_t)r * nb + blk) * 9 + k];
+      for (int k = 0; k < 9; ++k) v[k] += part[((size_t)r * stride + blk) * 9 + k];
     const double vol = (double)b->reps[r].volume;
     out[r].energy = v[0] / vol;
     out[r].magnetization_squared = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
@@ -576,11 +607,6 @@ int slsf_get_acceptance(slsf_batch* b, double* acceptance) {
 // ---------------------------------------------------------------------------
 // production runs
 // ---------------------------------------------------------------------------
-namespace slsf {
-void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t bin_cap, hipStream_t s);
-}
-
 namespace {
 
 int ensure_bins(slsf_batch* b, const slsf_run_options* opt) {
@@ -680,13 +706,9 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     if (int rc = resident_run(b->resident, b->g, a, b->stream, &res)) return rc;
     launches = res.launches;
     kernel_ms = res.kernel_ms;
-    std::vector<unsigned long long> acc(b->reps.size());
-    std::vector<long long> cs(b->reps.size());
-    HIP_TRY(hipMemcpyAsync(acc.data(), b->g.acc, sizeof(unsigned long long) * acc.size(),
-                           hipMemcpyDeviceToHost, b->stream));
-    HIP_TRY(hipMemcpyAsync(cs.data(), b->g.csize, sizeof(long long) * cs.size(),
-                           hipMemcpyDeviceToHost, b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<unsigned long long> acc, cs;
+    if (int rc = read_tallies(b, b->g.acc, acc)) return rc;
+    if (int rc = read_tallies(b, (unsigned long long*)b->g.csize, cs)) return rc;
     const int64_t nsw = opt->thermalization + opt->sweeps;
     for (size_t r = 0; r < b->reps.size(); ++r) {
       b->acc_sum[r] += (double)acc[r] / (double)b->reps[r].volume;
@@ -695,24 +717,29 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
       b->cluster_count[r] += nsw * b->reps[r].wolff_steps;
     }
   } else {
+    // HBM path: tallies stay on the device until the end of the run
     const int wmax = max_wolff_steps(b);
     const int64_t total = opt->thermalization + opt->sweeps;
+    launch_clear_counters(b->g, b->stream);
+    launch_sweep_end(b->g, 0, 0, b->stream);
+    launches += 3;
     for (int64_t s = 0; s < total; ++s) {
-      if (int rc = metro_sweep(b, nullptr)) return rc;
-      if (int rc = gather_acceptance(b, nullptr)) return rc;
-      for (int k = 0; k < wmax; ++k) {
+      if (int rc = metro_phases(b, nullptr)) return rc;
+      for (int k = 0; k < wmax; ++k)
         if (int rc = wolff_round(b, k, nullptr)) return rc;
-        if (int rc = gather_clusters(b, k, nullptr)) return rc;
-      }
-      launches += 4 + 5 * wmax;
+      launches += 2 + 4 * wmax;
       if (s >= opt->thermalization) {
         if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
         else launch_observables<float>(b->g, b->stream);
         launch_accumulate(b->g, b->d_accum, b->d_bins, b->d_bin_state, b->bin_cap, b->stream);
         launches += 2;
-        HIP_TRY(hipGetLastError());
       }
+      launch_sweep_end(b->g, 1, wmax, b->stream);
+      launches += 1;
+      HIP_TRY(hipGetLastError());
     }
+    if (int rc = gather_acceptance(b, total, nullptr)) return rc;
+    if (int rc = gather_clusters(b, clusters_in(b, total), nullptr)) return rc;
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   const auto t1 = std::chrono::steady_clock::now();

@@ -1,4 +1,7 @@
-// FP32 plain-path kernels (state in HBM, one thread per site).
+// FP32 plain-path kernels (state in HBM, one thread per site), plus the
+// type-independent launchers of the HBM path.
+#include <algorithm>
+
 #include "launch.h"
 #include "global_kernels.inc"
 #define SLSF_REAL float
@@ -6,31 +9,78 @@
 #undef SLSF_REAL
 
 namespace slsf {
-void launch_clear_counters(const GlobalState& g, hipStream_t s) {
-  k_fill_u64<<<(g.nrep + 255) / 256, 256, 0, s>>>(g.acc, g.nrep);
+
+// sweep bookkeeping of the HBM path: a replica that took part adds `metro` to
+// its Metropolis counter and its own number of cluster rounds (at most
+// `rounds`) to its cluster counter; then every replica is marked for the next
+// sweep iff its counter is below its window end.  (metro, rounds) = (0, 0)
+// only sets the marks, which is how a run starts.
+__global__ void k_sweep_end(RepDev* reps, int nrep, int metro, int rounds) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  RepDev& d = reps[r];
+  if (d.active) {
+    d.metro_ctr += (uint32_t)metro;
+    d.wolff_ctr += (uint32_t)(rounds < d.wolff_steps ? rounds : d.wolff_steps);
+  }
+  d.active = d.metro_ctr < d.stop_at ? 1u : 0u;
 }
-}  // namespace slsf
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+void set_extents(GlobalState& g, const RepDev* reps, int n) {
+  g.max_volume = 0;
+  g.max_plane = 0;
+  g.max_lz = 0;
+  g.obs_blocks = 0;
+  g.max_tiles = 0;
+  for (int r = 0; r < n; ++r) {
+    g.max_volume = std::max<int64_t>(g.max_volume, reps[r].volume);
+    g.max_plane = std::max<int64_t>(g.max_plane, (int64_t)reps[r].lx * reps[r].ly);
+    g.max_lz = std::max(g.max_lz, reps[r].lz);
+    g.obs_blocks = std::max(g.obs_blocks, obs_blocks_of(reps[r].lx, reps[r].ly, reps[r].lz));
+    g.max_tiles = std::max(g.max_tiles, wolff_tiles_of(reps[r].lx, reps[r].ly, reps[r].lz));
+  }
+}
+
+void launch_clear_counters(const GlobalState& g, hipStream_t s) {
+  const int64_t n = (int64_t)g.nrep * TALLY_SLOTS;
+  k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g.acc, n);
+  k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((unsigned long long*)g.csize, n);
+}
+
+void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s) {
+  k_sweep_end<<<(g.nrep + 63) / 64, 64, 0, s>>>(g.reps, g.nrep, metro, rounds);
+}
 
 // ---------------------------------------------------------------------------
 // per-measurement accumulation of the observable series into FP64 bins, on the
-// device: one thread per replica folds the block partials of k_observables in
-// block order, forms the 13 series values and appends a bin mean every
-// `binsize` measurements.
+// device: one wave per replica folds the block partials of k_observables (lane
+// k takes blocks k, k+64, ... in order, then a fixed shuffle tree), forms the
+// 13 series values and appends a bin mean every `binsize` measurements.  A
+// replica is measured iff it takes part in this sweep and its counter (not yet
+// advanced) is at least measure_from.
 // ---------------------------------------------------------------------------
-namespace slsf {
-
-__global__ void k_accumulate(const RepDev* __restrict__ reps, int nrep,
-                             const double* __restrict__ partials, int obs_blocks, double* accum,
-                             double* bins, int32_t* bin_state, int32_t bin_cap) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrep) return;
-  const RepDev d = reps[r];
-  // measured iff this sweep took part and its index metro_ctr - 1 >= measure_from
-  if (!d.active || d.metro_ctr == 0u || d.metro_ctr - 1u < d.measure_from) return;
-  const int32_t binsize = d.binsize;
+__global__ void __launch_bounds__(64)
+k_accumulate(const RepDev* __restrict__ reps, const double* __restrict__ partials,
+             int obs_stride, double* accum, double* bins, int32_t* bin_state, int32_t bin_cap) {
+  const int r = blockIdx.x;
+  const RepDev& d = reps[r];
+  if (!d.active || d.metro_ctr < d.measure_from) return;
+  const int lane = threadIdx.x;
+  const int nb = obs_blocks_of(d.lx, d.ly, d.lz);
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < obs_blocks; ++b)
-    for (int k = 0; k < 9; ++k) v[k] += partials[((int64_t)r * obs_blocks + b) * 9 + k];
+  for (int b = lane; b < nb; b += 64)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] += partials[((int64_t)r * obs_stride + b) * 9 + k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if (lane != 0) return;
+  const int32_t binsize = d.binsize;
   const double vol = (double)d.volume;
   const double e = v[0] / vol;
   const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
@@ -52,8 +102,8 @@ __global__ void k_accumulate(const RepDev* __restrict__ reps, int nrep,
 
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
                        int32_t bin_cap, hipStream_t s) {
-  k_accumulate<<<(g.nrep + 63) / 64, 64, 0, s>>>(g.reps, g.nrep, g.partials, g.obs_blocks,
-                                                  accum, bins, bin_state, bin_cap);
+  k_accumulate<<<g.nrep, 64, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins, bin_state,
+                                     bin_cap);
 }
 
 }  // namespace slsf

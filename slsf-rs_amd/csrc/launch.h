@@ -19,31 +19,41 @@ struct GlobalState {
   void* sth = nullptr;
   void* jxy = nullptr;      // R[total_layers]
   void* jz = nullptr;
-  uint8_t* bondmask = nullptr;        // [total_sites]
-  uint32_t* visited = nullptr;        // [total_sites / 32]
-  int32_t* qa = nullptr;              // [total_sites]
-  int32_t* qb = nullptr;
+  int max_lz = 0;           // largest replica extents (grid sizes)
+  int64_t max_plane = 0;
+  int max_tiles = 0;        // largest wolff_tiles_of
+  uint8_t* bondmask = nullptr;        // [total_sites] active-bond bits of the base site
+  uint32_t* visited = nullptr;        // [total_sites / 32] cluster membership bitmap
+  uint32_t* label = nullptr;          // [total_sites] union-find forest of the cluster step
   WolffDev* wd = nullptr;             // [nrep]
-  long long* csize = nullptr;         // [nrep]
-  unsigned long long* acc = nullptr;  // [nrep]
+  long long* csize = nullptr;         // [nrep * TALLY_SLOTS] cluster sites, summed until cleared
+  unsigned long long* acc = nullptr;  // [nrep * TALLY_SLOTS] accepted moves, summed until cleared
   double* partials = nullptr;         // [nrep * obs_blocks * 9]
-  int obs_blocks = 0;
+  int obs_blocks = 0;                 // max over replicas of obs_blocks_of
 };
+
+// grid extents of a GlobalState from its replica descriptors (host side)
+void set_extents(GlobalState& g, const RepDev* reps_host, int n);
 
 template <typename R> void launch_init(const GlobalState& g, int cold, hipStream_t s);
 template <typename R> void launch_refresh(const GlobalState& g, hipStream_t s);
-// one red/black sweep of every replica, then the sweep counters advance
-template <typename R> void launch_metro_sweep(const GlobalState& g, uint8_t* decisions,
-                                              hipStream_t s);
-// cluster round `round` (replicas with wolff_steps <= round sit it out):
-// seed + bond activations + BFS; flip/clear/advance follow separately so that
-// a caller can read the membership bitmap in between.
+// A sweep of the HBM path: launch_sweep_end(g, 0, 0) marks the replicas whose
+// window is open, then per sweep launch_metro_phases, launch_wolff for rounds
+// 0..wmax-1, optionally launch_observables + launch_accumulate, and
+// launch_sweep_end(g, 1, wmax) advances the counters and marks the next sweep.
+template <typename R> void launch_metro_phases(const GlobalState& g, uint8_t* decisions,
+                                               hipStream_t s);
+// cluster round `round` (replicas with wolff_steps <= round sit it out): seed,
+// bond activations, union-find, reflection; the membership bitmap stays valid
+// until the next round.
 template <typename R> void launch_wolff(const GlobalState& g, int round, hipStream_t s);
-template <typename R> void launch_wolff_flip_clear(const GlobalState& g, int round,
-                                                   hipStream_t s);
+void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s);
 template <typename R> void launch_observables(const GlobalState& g, hipStream_t s);
 template <typename R> void launch_corr(const GlobalState& g, int replica, int rxy, int rz,
                                        double* out_xy, double* out_z, hipStream_t s);
+// zeroes the accepted-move and cluster-site tallies
 void launch_clear_counters(const GlobalState& g, hipStream_t s);
+void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
+                       int32_t bin_cap, hipStream_t s);
 
 }  // namespace slsf

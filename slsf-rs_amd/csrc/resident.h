@@ -39,8 +39,8 @@ struct ResidentRunArgs {
   double* bins = nullptr;
   int32_t* bin_state = nullptr;
   int32_t bin_cap = 0;
-  unsigned long long* acc_out = nullptr;  // accepted proposals per replica (summed)
-  long long* csize_out = nullptr;         // cluster sites per replica (summed)
+  unsigned long long* acc_out = nullptr;  // accepted proposals, [replica * TALLY_SLOTS] (summed)
+  long long* csize_out = nullptr;         // cluster sites, [replica * TALLY_SLOTS] (summed)
 };
 
 struct ResidentRunResult {

@@ -442,8 +442,8 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   __syncthreads();
   wg_sum<2, S::NW>(tallies, red);
   if (t == 0) {
-    a.acc_out[rep] += (unsigned long long)tallies[0];
-    a.csize_out[rep] += (long long)tallies[1];
+    a.acc_out[(int64_t)rep * TALLY_SLOTS] += (unsigned long long)tallies[0];
+    a.csize_out[(int64_t)rep * TALLY_SLOTS] += (long long)tallies[1];
     a.reps[rep].metro_ctr = mctr;
     a.reps[rep].wolff_ctr = wctr;
   }
@@ -589,7 +589,8 @@ int resident_run(ResidentPlan& plan, const GlobalState& g, const ResidentRunArgs
     RES_TRY(hipEventCreate(&plan.t0), "hipEventCreate failed");
     RES_TRY(hipEventCreate(&plan.t1), "hipEventCreate failed");
   }
-  k_zero_tallies<<<(g.nrep + 255) / 256, 256, 0, s>>>(ra.acc_out, ra.csize_out, g.nrep);
+  k_zero_tallies<<<(g.nrep * TALLY_SLOTS + 255) / 256, 256, 0, s>>>(ra.acc_out, ra.csize_out,
+                                                                    g.nrep * TALLY_SLOTS);
   RES_TRY(hipEventRecord(plan.t0, s), "hipEventRecord failed");
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -666,8 +667,8 @@ int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
   int rc = copy_all(true) == hipSuccess ? SLSF_OK : fail(SLSF_E_DEVICE, "autotune save failed");
   unsigned long long* acc = nullptr;
   long long* cs = nullptr;
-  if (!rc && (hipMalloc((void**)&acc, g.nrep * 8) != hipSuccess ||
-              hipMalloc((void**)&cs, g.nrep * 8) != hipSuccess))
+  if (!rc && (hipMalloc((void**)&acc, (size_t)g.nrep * TALLY_SLOTS * 8) != hipSuccess ||
+              hipMalloc((void**)&cs, (size_t)g.nrep * TALLY_SLOTS * 8) != hipSuccess))
     rc = fail(SLSF_E_DEVICE, "hipMalloc failed");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc) {

@@ -26,11 +26,6 @@
 #include "launch.h"
 #include "resident.h"
 
-namespace slsf {
-void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t bin_cap, hipStream_t s);
-}
-
 using namespace slsf;
 
 namespace {
@@ -49,12 +44,13 @@ int64_t padded_volume(int L) { return ((int64_t)L * L * L + 63) / 64 * 64; }
 
 // device bytes of one slot: angles + trig cache (12 B/site), couplings,
 // descriptor, accumulators and bins; the HBM path adds its Wolff scratch
-// (bond masks 1 B, BFS queues 8 B, visited bits) and block partials
+// (bond masks 1 B, union-find labels 4 B, membership bits) and block partials
 int64_t slot_bytes(int L) {
   const int64_t vp = padded_volume(L);
   int64_t b = vp * 12 + 2 * L * 4 + (int64_t)sizeof(RepDev) + 13 * 8 + 2 * 4 +
-              13 * 8 * (int64_t)kBinCap + 16 + 8 + 8;
-  if (!resident_supports(L)) b += vp * 9 + vp / 8 + ((vp + 255) / 256) * 9 * 8 + (int64_t)sizeof(WolffDev);
+              13 * 8 * (int64_t)kBinCap + 2 * 8 * TALLY_SLOTS + 4;
+  if (!resident_supports(L))
+    b += vp * 5 + vp / 8 + (int64_t)obs_blocks_of(L, L, L) * 9 * 8 + (int64_t)sizeof(WolffDev);
   return b;
 }
 
@@ -145,9 +141,12 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   SRV_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
   GlobalState& g = p->g;
   g.nrep = cap;
-  g.max_volume = (int64_t)L * L * L;
   g.total_sites = p->vp * cap;
-  g.obs_blocks = (int)((g.max_volume + 255) / 256);
+  RepDev shape;
+  std::memset(&shape, 0, sizeof shape);
+  shape.lx = shape.ly = shape.lz = L;
+  shape.volume = (int64_t)L * L * L;
+  set_extents(g, &shape, 1);
   const size_t n = (size_t)g.total_sites;
   int rc = SLSF_OK;
   rc = rc ? rc : p->alloc((RepDev**)&g.reps, (size_t)cap);
@@ -156,8 +155,8 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   rc = rc ? rc : p->alloc((float**)&g.sth, n);
   rc = rc ? rc : p->alloc((float**)&g.jxy, (size_t)cap * L);
   rc = rc ? rc : p->alloc((float**)&g.jz, (size_t)cap * L);
-  rc = rc ? rc : p->alloc(&g.acc, (size_t)cap);
-  rc = rc ? rc : p->alloc(&g.csize, (size_t)cap);
+  rc = rc ? rc : p->alloc(&g.acc, (size_t)cap * TALLY_SLOTS);
+  rc = rc ? rc : p->alloc(&g.csize, (size_t)cap * TALLY_SLOTS);
   rc = rc ? rc : p->alloc(&p->accum, (size_t)cap * 13);
   rc = rc ? rc : p->alloc(&p->bin_state, (size_t)cap * 2);
   rc = rc ? rc : p->alloc(&p->bins, (size_t)cap * 13 * kBinCap);
@@ -165,8 +164,7 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   if (!p->resident) {
     rc = rc ? rc : p->alloc(&g.bondmask, n);
     rc = rc ? rc : p->alloc(&g.visited, n / 32);
-    rc = rc ? rc : p->alloc(&g.qa, n);
-    rc = rc ? rc : p->alloc(&g.qb, n);
+    rc = rc ? rc : p->alloc(&g.label, n);
     rc = rc ? rc : p->alloc(&g.wd, (size_t)cap);
     rc = rc ? rc : p->alloc(&g.partials, (size_t)cap * g.obs_blocks * 9);
   }
@@ -198,6 +196,7 @@ int admit(slsf_server* s, Pool& p, int slot, Job& j) {
   d.binsize = j.binsize;
   d.beta = 1.0 / q.temperature;
   d.width = q.proposal_width;
+  set_divisors(d);
   p.host[slot] = d;
   // couplings: J_xy[z] ~ U[min, max] from SplitMix64(disorder_seed), J_z = j_perp
   std::vector<float> jxy(L), jz(L);
@@ -214,8 +213,10 @@ int admit(slsf_server* s, Pool& p, int slot, Job& j) {
                          hipMemcpyHostToDevice, p.stream));
   SRV_TRY(hipMemsetAsync(p.accum + (size_t)slot * 13, 0, 13 * sizeof(double), p.stream));
   SRV_TRY(hipMemsetAsync(p.bin_state + (size_t)slot * 2, 0, 2 * sizeof(int32_t), p.stream));
-  SRV_TRY(hipMemsetAsync(p.g.acc + slot, 0, sizeof(unsigned long long), p.stream));
-  SRV_TRY(hipMemsetAsync(p.g.csize + slot, 0, sizeof(long long), p.stream));
+  SRV_TRY(hipMemsetAsync(p.g.acc + (size_t)slot * TALLY_SLOTS, 0,
+                         sizeof(unsigned long long) * TALLY_SLOTS, p.stream));
+  SRV_TRY(hipMemsetAsync(p.g.csize + (size_t)slot * TALLY_SLOTS, 0,
+                         sizeof(long long) * TALLY_SLOTS, p.stream));
   GlobalState one = p.g;  // keyed initial angles of this slot only
   one.reps = p.g.reps + slot;
   one.nrep = 1;
@@ -256,14 +257,13 @@ int run_chunk(slsf_server* s, Pool& p) {
     for (int i : p.active_idx) wmax = std::max(wmax, p.host[i].wolff_steps);
     GlobalState g = p.g;  // launch only up to the highest active slot
     g.nrep = p.active_idx.back() + 1;
+    launch_sweep_end(g, 0, 0, p.stream);  // marks the slots whose window is open
     for (int k = 0; k < s->chunk; ++k) {
-      launch_metro_sweep<float>(g, nullptr, p.stream);
-      for (int r = 0; r < wmax; ++r) {
-        launch_wolff<float>(g, r, p.stream);
-        launch_wolff_flip_clear<float>(g, r, p.stream);
-      }
+      launch_metro_phases<float>(g, nullptr, p.stream);
+      for (int r = 0; r < wmax; ++r) launch_wolff<float>(g, r, p.stream);
       launch_observables<float>(g, p.stream);
       launch_accumulate(g, p.accum, p.bins, p.bin_state, kBinCap, p.stream);
+      launch_sweep_end(g, 1, wmax, p.stream);
     }
   }
   SRV_TRY(hipGetLastError());
@@ -274,10 +274,12 @@ int retire(slsf_server* s, Pool& p, int32_t* nfinished) {
   if (!p.running) return SLSF_OK;
   SRV_TRY(hipStreamSynchronize(p.stream));
   std::vector<RepDev> devs(p.cap);
-  std::vector<unsigned long long> acc(p.cap);
+  std::vector<unsigned long long> slots((size_t)p.cap * TALLY_SLOTS), acc(p.cap, 0ull);
   SRV_TRY(hipMemcpy(devs.data(), p.g.reps, sizeof(RepDev) * p.cap, hipMemcpyDeviceToHost));
-  SRV_TRY(hipMemcpy(acc.data(), p.g.acc, sizeof(unsigned long long) * p.cap,
+  SRV_TRY(hipMemcpy(slots.data(), p.g.acc, sizeof(unsigned long long) * slots.size(),
                     hipMemcpyDeviceToHost));
+  for (int i = 0; i < p.cap; ++i)
+    for (int k = 0; k < TALLY_SLOTS; ++k) acc[i] += slots[(size_t)i * TALLY_SLOTS + k];
   const double t = now_since(s->t0);
   for (int slot : p.active_idx) {
     const RepDev& d = devs[slot];

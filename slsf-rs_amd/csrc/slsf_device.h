@@ -167,6 +167,21 @@ SLSF_HD float wrap_angle(float t) {
   return t;
 }
 
+// ---- exact division by a lattice extent -----------------------------------
+// floor(n / d) for 32-bit n and d as a multiply-high by m = ceil(2^64 / d)
+// (m = 0 encodes d = 1).  With n = q d + r, n m / 2^64 = n / d + n e / (d 2^64)
+// for some 0 <= e < d, and the excess stays below 1 - r/d because n < 2^32 and
+// d < 2^32, so the floor is q exactly.
+SLSF_HD uint64_t divisor_magic(uint32_t d) {
+  return d <= 1u ? 0ull : (~0ull) / d + 1ull;  // floor((2^64 - 1) / d) + 1 = ceil(2^64 / d)
+}
+SLSF_HD uint32_t fast_div(uint32_t n, uint64_t m) {
+  if (m == 0ull) return n;
+  const uint64_t lo = (uint64_t)n * (uint32_t)m;
+  const uint64_t hi = (uint64_t)n * (uint32_t)(m >> 32);
+  return (uint32_t)((hi + (lo >> 32)) >> 32);
+}
+
 // ---- per-replica descriptor shared by the global-memory kernels ------------
 struct RepDev {
   int32_t lx, ly, lz, wolff_steps;
@@ -178,12 +193,33 @@ struct RepDev {
   uint32_t wolff_ctr;  // Wolff clusters done (counter of TAG_WSEED/TAG_WBOND)
   uint32_t measure_from;  // sweeps with counter >= this are measured
   uint32_t stop_at;       // the replica does no sweep once metro_ctr reaches this
-  uint32_t active;        // set per sweep by the HBM path: metro_ctr < stop_at
+  uint32_t active;        // HBM path: this replica takes part in the current sweep
   int32_t binsize;        // measurements per bin
-  int32_t pad[2];
+  int32_t pad0;
   double beta, width;
+  uint64_t div_lx, div_hx, div_plane;  // divisor_magic of lx, lx/2, lx*ly
+  uint64_t pad1[2];
 };
-static_assert(sizeof(RepDev) == 96, "RepDev layout");
+static_assert(sizeof(RepDev) == 128, "RepDev layout");
+
+// fills the divisor fields from lx, ly (host side, after the sizes are set)
+inline void set_divisors(RepDev& d) {
+  d.div_lx = divisor_magic((uint32_t)d.lx);
+  d.div_hx = divisor_magic((uint32_t)(d.lx / 2));
+  d.div_plane = divisor_magic((uint32_t)d.lx * (uint32_t)d.ly);
+}
+
+// Per-replica tallies (accepted moves, cluster sites) are spread over
+// TALLY_SLOTS counters so that the blocks of one launch do not all update one
+// address; a tally is the sum of its replica's slots.
+constexpr int TALLY_SLOTS = 64;
+
+// block partition of the fused observables: 2048 sites (8 per thread of a
+// 256-thread block) per block, blocks never straddle planes; a function of the
+// replica's own sizes only, so its FP64 sums do not depend on the batch
+constexpr int OBS_THREADS = 256, OBS_PER_THREAD = 8, OBS_CHUNK = OBS_THREADS * OBS_PER_THREAD;
+SLSF_HD int obs_chunks_per_plane(int lx, int ly) { return (lx * ly + OBS_CHUNK - 1) / OBS_CHUNK; }
+SLSF_HD int obs_blocks_of(int lx, int ly, int lz) { return lz * obs_chunks_per_plane(lx, ly); }
 
 struct WolffDev {  // per replica, per cluster step
   double phi, sphi, cphi;
