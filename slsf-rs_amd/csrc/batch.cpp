@@ -717,27 +717,46 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
       b->cluster_count[r] += nsw * b->reps[r].wolff_steps;
     }
   } else {
-    // HBM path: tallies stay on the device until the end of the run
+    // HBM path: tallies stay on the device until the end of the run; the
+    // replicas' windows cut thermalization (no measurement) and the tail
     const int wmax = max_wolff_steps(b);
     const int64_t total = opt->thermalization + opt->sweeps;
     launch_clear_counters(b->g, b->stream);
     launch_sweep_end(b->g, 0, 0, b->stream);
     launches += 3;
-    for (int64_t s = 0; s < total; ++s) {
-      if (int rc = metro_phases(b, nullptr)) return rc;
-      for (int k = 0; k < wmax; ++k)
-        if (int rc = wolff_round(b, k, nullptr)) return rc;
-      launches += 2 + 4 * wmax;
-      if (s >= opt->thermalization) {
-        if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
-        else launch_observables<float>(b->g, b->stream);
-        launch_accumulate(b->g, b->d_accum, b->d_bins, b->d_bin_state, b->bin_cap, b->stream);
-        launches += 2;
-      }
-      launch_sweep_end(b->g, 1, wmax, b->stream);
-      launches += 1;
-      HIP_TRY(hipGetLastError());
+    BinTarget bt;
+    if (opt->sweeps > 0) {
+      bt.accum = b->d_accum;
+      bt.bins = b->d_bins;
+      bt.bin_state = b->d_bin_state;
+      bt.bin_cap = b->bin_cap;
     }
+    auto enqueue = [&](int64_t n) {
+      return is_fp64(b) ? enqueue_hbm_sweeps<double>(b->g, n, wmax, bt, b->stream)
+                        : enqueue_hbm_sweeps<float>(b->g, n, wmax, bt, b->stream);
+    };
+    if (opt->use_graph && total > 0) {
+      // one graph of `chunk` sweeps, replayed; sweeps past the window are no-ops
+      const int64_t chunk = std::min<int64_t>(
+          total, opt->sweeps_per_launch > 0 ? opt->sweeps_per_launch : 20);
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int64_t per = enqueue(chunk);
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      for (int64_t done = 0; e == hipSuccess && done < total; done += chunk) {
+        e = hipGraphLaunch(exec, b->stream);
+        launches += per;
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      HIP_TRY(e);
+    } else {
+      launches += enqueue(total);
+    }
+    HIP_TRY(hipGetLastError());
     if (int rc = gather_acceptance(b, total, nullptr)) return rc;
     if (int rc = gather_clusters(b, clusters_in(b, total), nullptr)) return rc;
   }

@@ -48,12 +48,29 @@ template <typename R> void launch_metro_phases(const GlobalState& g, uint8_t* de
 // until the next round.
 template <typename R> void launch_wolff(const GlobalState& g, int round, hipStream_t s);
 void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s);
-template <typename R> void launch_observables(const GlobalState& g, hipStream_t s);
+// windowed = 1: only replicas measured in this sweep (active, counter at least
+// measure_from) compute their partials; 0: every replica (a direct measurement)
+template <typename R> void launch_observables(const GlobalState& g, hipStream_t s,
+                                              int windowed = 0);
 template <typename R> void launch_corr(const GlobalState& g, int replica, int rxy, int rz,
                                        double* out_xy, double* out_z, hipStream_t s);
 // zeroes the accepted-move and cluster-site tallies
 void launch_clear_counters(const GlobalState& g, hipStream_t s);
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
                        int32_t bin_cap, hipStream_t s);
+
+// device bins of a measured run; accum == nullptr runs without measurement
+struct BinTarget {
+  double* accum = nullptr;
+  double* bins = nullptr;
+  int32_t* bin_state = nullptr;
+  int32_t bin_cap = 0;
+};
+// Enqueues `nsweeps` sweeps of the HBM path on s: Metropolis, `wmax` cluster
+// rounds, windowed observables + accumulation, counters.  Every sweep issues
+// the same launches (the replicas' windows decide who takes part), so the
+// sequence can be captured into a graph and replayed.  Returns the launch count.
+template <typename R> int64_t enqueue_hbm_sweeps(const GlobalState& g, int64_t nsweeps, int wmax,
+                                                 const BinTarget& bt, hipStream_t s);
 
 }  // namespace slsf

@@ -258,13 +258,12 @@ int run_chunk(slsf_server* s, Pool& p) {
     GlobalState g = p.g;  // launch only up to the highest active slot
     g.nrep = p.active_idx.back() + 1;
     launch_sweep_end(g, 0, 0, p.stream);  // marks the slots whose window is open
-    for (int k = 0; k < s->chunk; ++k) {
-      launch_metro_phases<float>(g, nullptr, p.stream);
-      for (int r = 0; r < wmax; ++r) launch_wolff<float>(g, r, p.stream);
-      launch_observables<float>(g, p.stream);
-      launch_accumulate(g, p.accum, p.bins, p.bin_state, kBinCap, p.stream);
-      launch_sweep_end(g, 1, wmax, p.stream);
-    }
+    BinTarget bt;
+    bt.accum = p.accum;
+    bt.bins = p.bins;
+    bt.bin_state = p.bin_state;
+    bt.bin_cap = kBinCap;
+    (void)enqueue_hbm_sweeps<float>(g, s->chunk, wmax, bt, p.stream);
   }
   SRV_TRY(hipGetLastError());
   return SLSF_OK;

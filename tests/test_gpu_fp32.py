@@ -256,3 +256,26 @@ def test_graph_capture_matches_stream_launches():
                     np.stack([b.all_bins(i) for i in range(len(reps))])))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision", [slsf.FP64_REPLAY, slsf.FP32])
+def test_hbm_graph_replay_matches_stream_launches(precision):
+    """HBM-path runs (sizes the resident kernel does not take, mixed in one
+    batch) give the same angles and bins with and without graph replay, also
+    when the graph's sweep chunk does not divide the run (the replicas' windows
+    cut the last replay)."""
+    reps = reps_for(20, 2, T=1.1, seed0=40) + reps_for(10, 3, T=1.6, seed0=50)
+    reps += [slsf.Replica(6, 10, 14, 0.9, np.linspace(0.6, 1.8, 14), np.ones(14), 77,
+                          wolff_steps=2)]
+    out = []
+    for graph in (False, True):
+        b = slsf.Batch(reps, precision)
+        b.init_angles()
+        b.run(7, 0, 1, sweeps_per_launch=3, use_graph=graph)
+        b.run(5, 16, 4, sweeps_per_launch=6, use_graph=graph)
+        out.append((np.concatenate([b.get_theta(i) for i in range(len(reps))]),
+                    np.stack([b.all_bins(i) for i in range(len(reps))]), b.counters()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2][0], out[1][2][0]) and np.array_equal(out[0][2][1], out[1][2][1])
+    assert list(out[0][2][0]) == [28] * len(reps)
