@@ -1,20 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark: Metropolis+Wolff site-updates/s on one MI355X.
 
-Workload (default): the batched disorder average -- L=32, 256 disorder
-realizations x 8 temperatures T in {1.10, 1.15, ..., 1.45} = 2048 replicas,
-J_xy[z] ~ U[0.6, 1.8] iid per layer from the realization's seed, J_perp = 1.0,
-proposal width pi, one Wolff cluster per sweep, FP32 sweeps with FP64
-accumulators, one measurement (13 series) per sweep binned on the device.
-A step is one sweep of all 2048 replicas (Metropolis + Wolff + measurement).
+Default workload (the headline line) -- the batched disorder average: L=32,
+256 disorder realizations x 8 temperatures T in {1.10, 1.15, ..., 1.45} = 2048
+replicas, J_xy[z] ~ U[0.6, 1.8] iid per layer from the realization's seed,
+J_perp = 1.0, proposal width pi, one Wolff cluster per sweep, FP32 sweeps with
+FP64 accumulators, one measurement (13 series) per sweep binned on the device;
+2000 thermalization sweeps (the untimed warmup) + 10000 timed sweeps.  A step
+is one sweep of all 2048 replicas (Metropolis + Wolff + measurement).
 
-Also reported: the plain single-replica GPU path (one L=32 replica on the HBM
-kernels) and its ratio to the batched form; a roofline object for the dominant
-kernel; and a CPU baseline (the oracle's reference-mode restatement of the
-reference CPU path, timed on this host's cores on a bounded sample).
+The same line reports the plain single-replica GPU path (one L=32 replica on
+the HBM kernels) and the batched/plain ratio, Metropolis-only and Wolff-only
+rates, the mean cluster size, a roofline object for the dominant kernel and a
+CPU baseline (the oracle's reference-mode restatement of the reference CPU
+path, timed on this host's cores on a bounded sample).
+
+Other workloads, one JSON line each (--workload):
+  c1  plain method: L=16, T=1.3, 1 replica, FP64 replay, 1000 + 5000 sweeps,
+      measured every sweep, C(r) along x up to r = 8 (run_theta_simulation)
+  c2  single replica at scale: L=64, T=1.3, 1000 sweeps, FP64 replay then FP32
+  c4  uneven batch: L in {8,12,16,24,32} x T in {1.1,...,1.5} x 64 realizations,
+      autotuned, 1000 + 4000 sweeps; disorder-averaged Binder ratio and helicity
+      moduli with FP64 jackknife errors and Binder crossing temperatures
+  c5  serving: Poisson arrivals (50 jobs/s, seed 42) of jobs with L in
+      {16,32,48,64,96,128}, T ~ U[0.3,2.2], sweeps ~ U{1000..10000}; jobs are
+      admitted by device-memory footprint; p50/p99 latency and site-updates/s
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -26,6 +40,12 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+METRIC = "Metropolis+Wolff site-updates/s (batched vs plain, 1 GPU); job p50/p99 latency"
+DATA = ("synthetic: J_xy[z] ~ U[0.6,1.8] per layer from the realization seed, J_perp=1.0, "
+        "random initial angles")
+# per-launch HBM bytes of the headline kernel, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (calibrated against a device copy of known size); written by the profiling run
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_resident_L32_2048replicas.json")
 
 
 def splitmix_uniform_layers(lz, seed, lo=0.6, hi=1.8):
@@ -45,16 +65,28 @@ def splitmix_uniform_layers(lz, seed, lo=0.6, hi=1.8):
     return out
 
 
-def c3_replicas(slsf, n_real=256, temps=None, L=32):
+def task_seed(base, L, sample, t):
+    """Replica seed of the job config's task expansion (sample counts from 1)."""
+    return base + 100000 * L + 1000 * sample + int(round(100 * t))
+
+
+def disorder_batch(slsf, Ls, temps, n_real, base_seed, wolff_steps=1):
+    """Replicas (L, realization, T) with the realization's couplings shared across T."""
+    reps, keys = [], []
+    for L in Ls:
+        for r in range(n_real):
+            jxy = splitmix_uniform_layers(L, base_seed + r)
+            jz = np.ones(L)
+            for t in temps:
+                reps.append(slsf.Replica(L, L, L, t, jxy, jz, task_seed(base_seed, L, r + 1, t),
+                                         float(np.pi), wolff_steps))
+                keys.append((L, r, t))
+    return reps, keys
+
+
+def c3_replicas(slsf, n_real=256, temps=None, L=32, wolff_steps=1):
     temps = temps if temps is not None else [1.10 + 0.05 * k for k in range(8)]
-    reps = []
-    for r in range(n_real):
-        jxy = splitmix_uniform_layers(L, 1000 + r)
-        jz = np.ones(L)
-        for t in temps:
-            seed = 1000 + 100000 * L + 1000 * (r + 1) + int(round(100 * t))
-            reps.append(slsf.Replica(L, L, L, t, jxy, jz, seed, float(np.pi), 1))
-    return reps
+    return disorder_batch(slsf, [L], temps, n_real, 1000, wolff_steps)[0]
 
 
 def cpu_baseline(budget_s=12.0):
@@ -62,8 +94,7 @@ def cpu_baseline(budget_s=12.0):
     Wolff DFS with trig cache, libm sincos/exp) on one L=32 replica per core."""
     import multiprocessing as mp
     cores = min(os.cpu_count() or 1, 16)
-    # calibrate one core
-    rate1 = _cpu_worker((0, 3))
+    rate1 = _cpu_worker((0, 3))  # calibrate one core
     sweeps = max(2, int(budget_s * rate1 / (32 ** 3)))
     t0 = time.perf_counter()
     with mp.get_context("fork").Pool(cores) as pool:
@@ -79,12 +110,12 @@ def cpu_baseline(budget_s=12.0):
 
 def _cpu_worker(arg):
     idx, sweeps = arg
+    import ctypes
     import oracle_lib as orc
     L = orc.load(fast=True)
     lat = orc.Lattice(32, 32, 32, lib_=L)
     lat.set_couplings(splitmix_uniform_layers(32, 1000 + idx), np.ones(32))
     r = orc.FastRng()
-    import ctypes
     L.or_rng_seed(ctypes.byref(r), 7 + idx)
     L.or_initialize_angles(lat.ref, 0, ctypes.byref(r))
     lat.refresh(False)
@@ -112,16 +143,249 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "GB_per_s_read_plus_write": 2 * nbytes / dt / 1e9}))
 
 
+def line(value, unit, steps, warmup, ms_per_step, workload, dtype, higher=True, **extra):
+    out = {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms_per_step, "higher_is_better": higher,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": DATA,
+           "config": workload}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def measured_traffic(spl, nrep):
+    """HBM bytes per launch of k_resident from the committed counter run, when
+    that run used the same launch shape; None otherwise."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("sweeps_per_launch") != spl or t.get("replicas") != nrep:
+        return None
+    return t["fetch_bytes"] + t["write_bytes"]
+
+
+# ---------------------------------------------------------------------------
+def bench_c3(args, slsf):
+    reps = c3_replicas(slsf, args.realizations)
+    V = 32 ** 3
+    b = slsf.Batch(reps, slsf.FP32)
+    b.init_angles()
+    b.run(args.warmup, 0, 1, sweeps_per_launch=args.sweeps_per_launch)
+    st = b.run(0, args.steps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
+    b.close()
+    updates = len(reps) * V * args.steps
+    value = updates / st["seconds"]
+
+    # Metropolis-only rate: the same batch with no cluster step; the Wolff
+    # share is the difference of the two times per sweep
+    msteps = max(100, args.steps // 10)
+    m = slsf.Batch(c3_replicas(slsf, args.realizations, wolff_steps=0), slsf.FP32)
+    m.init_angles()
+    m.run(min(args.warmup, 200), 0, 1, sweeps_per_launch=args.sweeps_per_launch)
+    sm = m.run(0, msteps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
+    m.close()
+    t_full = st["seconds"] / args.steps
+    t_metro = sm["seconds"] / msteps
+    metro_rate = len(reps) * V / t_metro
+    wolff_rate = len(reps) * V / (t_full - t_metro) if t_full > t_metro else None
+
+    plain = None
+    if not args.no_plain:
+        p = slsf.Batch(reps[:1], slsf.FP32)
+        p.init_angles()
+        p.run(5, 0, 1)
+        nplain = 200
+        sp = p.run(0, nplain, 10)
+        p.close()
+        plain = V * nplain / sp["seconds"]
+
+    spl = args.sweeps_per_launch or 50
+    kernel_ms = st.get("kernel_ms", 0.0)  # HIP events on the launch stream
+    launches = max(st["launches"], 1)
+    bytes_per_update = 8.0  # FP32 angle read + write per site update
+    alg_bytes_launch = len(reps) * V * bytes_per_update * spl
+    avg_launch_ms = kernel_ms / launches
+    achieved = (len(reps) * V * args.steps * bytes_per_update / (kernel_ms * 1e-3) / 1e9
+                if kernel_ms > 0 else value * bytes_per_update / 1e9)
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(spl, len(reps))}
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    line(value, "site-updates/s", args.steps, args.warmup, 1e3 * st["seconds"] / args.steps,
+         {"workload": f"L=32 batched disorder average: {args.realizations} realizations x "
+                      "8 T in [1.10,1.45], Metropolis + 1 Wolff + 13-series measurement "
+                      "per sweep, 2000 thermalization + 10000 measured sweeps",
+          "replicas": len(reps), "L": 32, "binsize": args.binsize},
+         "fp32 (fp64 accumulators)",
+         plain_single_replica=plain,
+         batched_over_plain=(value / plain) if plain else None,
+         metropolis_only_rate=metro_rate, wolff_only_rate=wolff_rate,
+         acceptance=st["acceptance_mean"], mean_cluster_size=st["mean_cluster_size"],
+         roofline=roofline, cpu_baseline=cpu,
+         kernel={"name": "k_resident<32, paired>", "launches": st["launches"],
+                 "avg_launch_ms": avg_launch_ms, "sweeps_per_launch": spl,
+                 "algorithmic_bytes_per_launch": alg_bytes_launch})
+
+
+def bench_c1(args, slsf):
+    L, T = 16, 1.3
+    jxy = splitmix_uniform_layers(L, 1)
+    t0 = time.perf_counter()
+    r = slsf.run_theta_simulation(L, L, L, 1.2, 1.0, 0.0, T, thermal_sweeps=1000,
+                                  measure_sweeps=5000, measure_interval=1, correlation_rmax=8,
+                                  seed=1, j_xy_array=jxy, j_z_array=np.ones(L),
+                                  precision=slsf.FP64_REPLAY)
+    dt = time.perf_counter() - t0
+    line(6000 * L ** 3 / dt, "site-updates/s", 5000, 1000, 1e3 * dt / 6000,
+         {"workload": "plain method: L=16, T=1.3, one replica, FP64 replay, 1000 + 5000 "
+                      "sweeps, measured every sweep, C(r) along x up to r=8"},
+         "fp64", energy=r["energy"], rho_sx=r["rho_sx"], rho_sz=r["rho_sz"],
+         magnetization_squared=r["magnetization_squared"], acceptance=r["acceptance"],
+         corr_x=list(map(float, r["corr_x"])))
+
+
+def bench_c2(args, slsf):
+    L, T = 64, 1.3
+    rep = slsf.Replica(L, L, L, T, splitmix_uniform_layers(L, 1), np.ones(L),
+                       task_seed(1, L, 1, T))
+    res = {}
+    for name, prec in (("fp64_replay", slsf.FP64_REPLAY), ("fp32", slsf.FP32)):
+        b = slsf.Batch([rep], prec)
+        b.init_angles()
+        b.run(20, 0, 1)
+        st = b.run(0, 1000, 10)
+        b.close()
+        res[name] = {"rate": L ** 3 * 1000 / st["seconds"], "seconds": st["seconds"],
+                     "acceptance": st["acceptance_mean"],
+                     "mean_cluster_size": st["mean_cluster_size"]}
+    f = res["fp32"]
+    line(f["rate"], "site-updates/s", 1000, 20, 1e3 * f["seconds"] / 1000,
+         {"workload": "single replica at scale: L=64, T=1.3, 1000 Metropolis + Wolff sweeps "
+                      "with measurement, HBM kernels"},
+         "fp32 (fp64 accumulators)", fp64_replay=res["fp64_replay"], fp32=f)
+
+
+def bench_c4(args, slsf):
+    Ls = [8, 12, 16, 24, 32]
+    temps = [1.1, 1.2, 1.3, 1.4, 1.5]
+    n_real = 64
+    reps, keys = disorder_batch(slsf, Ls, temps, n_real, 1)
+    b = slsf.Batch(reps, slsf.FP32)
+    b.init_angles()
+    tuned = b.autotune(16)
+    therm, sweeps, binsize = 1000, 4000, 100
+    b.run(therm, 0, 1)
+    st = b.run(0, sweeps, binsize)
+    updates = sum(r.volume for r in reps) * (therm + sweeps)
+    value = sum(r.volume for r in reps) * sweeps / st["seconds"]
+    # disorder averages per (L, T): [<m^4>] / [<m^2>]^2 and the helicity moduli,
+    # with jackknife errors over realizations
+    s_m2, s_m4 = slsf.SERIES.index("MagnetizationSquared"), slsf.SERIES.index("MagnetizationFourth")
+    per = {}
+    for i, (L, r, t) in enumerate(keys):
+        bins = b.all_bins(i)
+        ev = slsf.evaluate(bins, float(L ** 3), 1.0 / t)
+        per.setdefault((L, t), []).append((bins[s_m2].mean(), bins[s_m4].mean(),
+                                           ev["RhoXY"][0], ev["RhoZ"][0]))
+    b.close()
+    table = {}
+    for (L, t), rows in per.items():
+        a = np.array(rows)
+        m2, m4 = a[:, 0], a[:, 1]
+        g_all = m4.mean() / m2.mean() ** 2
+        n = len(a)
+        loo = [(m4.sum() - m4[k]) / (n - 1) / ((m2.sum() - m2[k]) / (n - 1)) ** 2 for k in range(n)]
+        g, ge = slsf.jackknife(np.array(loo), g_all)
+        rx = a[:, 2]
+        rz = a[:, 3]
+        table.setdefault(str(L), {})[f"{t:.1f}"] = {
+            "binder": g, "binder_err": ge,
+            "rho_xy": rx.mean(), "rho_xy_err": rx.std(ddof=1) / math.sqrt(n),
+            "rho_z": rz.mean(), "rho_z_err": rz.std(ddof=1) / math.sqrt(n)}
+    crossings = {}
+    for L1, L2 in zip(Ls[:-1], Ls[1:]):
+        d = [table[str(L1)][f"{t:.1f}"]["binder"] - table[str(L2)][f"{t:.1f}"]["binder"]
+             for t in temps]
+        for k in range(len(temps) - 1):
+            if d[k] == 0 or d[k] * d[k + 1] < 0:
+                tc = temps[k] + (temps[k + 1] - temps[k]) * d[k] / (d[k] - d[k + 1])
+                crossings[f"{L1}/{L2}"] = tc
+                break
+    line(value, "site-updates/s", sweeps, therm, 1e3 * st["seconds"] / sweeps,
+         {"workload": "uneven batch, autotuned: L in {8,12,16,24,32} x 5 T in [1.1,1.5] x 64 "
+                      "realizations (1600 replicas), 1000 + 4000 sweeps, binsize 100",
+          "replicas": len(reps)},
+         "fp32 (fp64 accumulators)", autotune=tuned, binder_rho=table,
+         binder_crossings=crossings, acceptance=st["acceptance_mean"],
+         mean_cluster_size=st["mean_cluster_size"], total_site_updates=updates)
+
+
+def bench_c5(args, slsf):
+    free, _total = slsf.device_memory()
+    budget = int(free * 0.9)
+    rng = np.random.default_rng(42)
+    lam, window = 50.0, args.serve_window
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / lam)
+        if t > window:
+            break
+        L = int(rng.choice([16, 32, 48, 64, 96, 128]))
+        T = float(rng.uniform(0.3, 2.2))
+        sw = int(rng.integers(1000, 10001))
+        seed = int(rng.integers(0, 2 ** 63))
+        arrivals.append((t, L, T, sw, seed))
+    srv = slsf.Server(budget, 256, chunk_sweeps=100)
+    i, res, steps = 0, [], 0
+    t0 = time.monotonic()
+    while True:
+        now = time.monotonic() - t0
+        while i < len(arrivals) and arrivals[i][0] <= now:
+            _, L, T, sw, seed = arrivals[i]
+            therm = sw // 10
+            srv.submit(l=L, temperature=T, seed=seed, disorder_seed=seed ^ 0x5DEECE66D,
+                       sweeps=sw - therm, thermalization=therm, binsize=max(1, (sw - therm) // 50))
+            i += 1
+        st = srv.stats()
+        if i == len(arrivals) and st["queued"] == 0 and st["active"] == 0:
+            break
+        if st["queued"] == 0 and st["active"] == 0:
+            time.sleep(max(0.0, arrivals[i][0] - now))
+            continue
+        srv.step()
+        steps += 1
+        res += srv.poll()
+    wall = time.monotonic() - t0
+    srv.close()
+    lat = np.array([r["finish_s"] - r["submit_s"] for r in res])
+    su = sum(r["site_updates"] for r in res)
+    by_l = {}
+    for r in res:
+        by_l.setdefault(r["l"], []).append(r["finish_s"] - r["submit_s"])
+    line(su / wall, "site-updates/s", steps, 0, 1e3 * wall / max(steps, 1),
+         {"workload": f"serving: Poisson arrivals 50 jobs/s for {window:g} s (seed 42), L in "
+                      "{16,32,48,64,96,128}, T ~ U[0.3,2.2], sweeps ~ U{1000..10000} "
+                      "(10% thermalization), chunks of 100 sweeps",
+          "jobs": len(arrivals), "device_budget_bytes": budget},
+         "fp32 (fp64 accumulators)",
+         latency_p50_s=float(np.percentile(lat, 50)), latency_p99_s=float(np.percentile(lat, 99)),
+         latency_p50_by_L={str(k): float(np.median(v)) for k, v in sorted(by_l.items())},
+         jobs_done=len(res), wall_s=wall,
+         offered_site_updates_per_s=sum(a[1] ** 3 * a[3] for a in arrivals) / window)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--binsize", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--binsize", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--realizations", type=int, default=256)
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
+    ap.add_argument("--serve-window", type=float, default=3.0)
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c3")
     ap.add_argument("--mode", choices=["bench", "copy"], default="bench",
                     help="copy: a plain 1 GiB device copy, to calibrate HBM byte counters")
     args = ap.parse_args()
@@ -134,56 +398,8 @@ def main():
         sys.exit(2)
 
     import slsf_amd as slsf
-
-    reps = c3_replicas(slsf, args.realizations)
-    b = slsf.Batch(reps, slsf.FP32)
-    b.init_angles()
-    b.run(args.warmup, 0, 1)
-    st = b.run(0, args.steps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
-    V = 32 ** 3
-    value = len(reps) * V * args.steps / st["seconds"]
-    ms_per_step = 1e3 * st["seconds"] / args.steps
-
-    plain = None
-    if not args.no_plain:
-        p = slsf.Batch(reps[:1], slsf.FP32)
-        p.init_angles()
-        p.run(5, 0, 1)
-        nplain = max(20, min(200, args.steps))
-        sp = p.run(0, nplain, 10)
-        plain = V * nplain / sp["seconds"]
-
-    kernel_ms = st.get("kernel_ms", 0.0)
-    bytes_per_update = 8.0  # FP32 angle read + write per site update
-    if kernel_ms > 0:
-        achieved = len(reps) * V * args.steps * bytes_per_update / (kernel_ms * 1e-3) / 1e9
-    else:
-        achieved = value * bytes_per_update / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
-    out = {
-        "metric": "Metropolis+Wolff site-updates/s (batched vs plain, 1 GPU)",
-        "value": value, "unit": "site-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (fp64 accumulators)",
-        "data": "synthetic: J_xy[z] ~ U[0.6,1.8] per realization seed, J_perp=1.0, "
-                "random initial angles",
-        "config": {"workload": f"L=32 batched disorder average: {args.realizations} "
-                               "realizations x 8 T in [1.10,1.45], Metropolis + 1 Wolff "
-                               "+ 13-series measurement per sweep",
-                   "replicas": len(reps), "L": 32, "binsize": args.binsize},
-        "plain_single_replica": plain,
-        "batched_over_plain": (value / plain) if plain else None,
-        "acceptance": st["acceptance_mean"], "mean_cluster_size": st["mean_cluster_size"],
-        "roofline": roofline, "cpu_baseline": cpu,
-        "kernel": {"name": "k_resident<32, paired>", "launches": st["launches"],
-                   "avg_launch_ms": kernel_ms / max(st["launches"], 1),
-                   "sweeps_per_launch": args.sweeps_per_launch or 50,
-                   "algorithmic_bytes_per_launch": len(reps) * V * bytes_per_update *
-                   (args.sweeps_per_launch or 50)},
-    }
-    print(json.dumps(out))
+    {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4,
+     "c5": bench_c5}[args.workload](args, slsf)
 
 
 if __name__ == "__main__":

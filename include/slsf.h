@@ -348,6 +348,9 @@ int slsf_server_poll(slsf_server* server, slsf_job_result* out, int32_t cap, int
 int slsf_server_get_stats(const slsf_server* server, slsf_server_stats* out);
 /* Bytes one slot of size l occupies (the admission footprint). */
 int64_t slsf_server_slot_bytes(int32_t l);
+/* Free and total memory of the current device (hipMemGetInfo), for choosing a
+ * server budget. */
+int slsf_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
 /* ======================================================================== */
 /* Job configuration (TOML) and task expansion                               */

@@ -329,6 +329,19 @@ extern "C" {
 
 int64_t slsf_server_slot_bytes(int32_t l) { return l > 0 ? slot_bytes(l) : 0; }
 
+int slsf_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  clear_error();
+  if (!free_bytes || !total_bytes) return fail(SLSF_E_INVALID, "null argument");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(SLSF_E_DEVICE, "no usable GPU (hipGetDeviceCount found none)");
+  size_t f = 0, t = 0;
+  SRV_TRY(hipMemGetInfo(&f, &t));
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
+  return SLSF_OK;
+}
+
 int slsf_server_create(int64_t budget, int32_t max_slots, int32_t chunk, slsf_server** out) {
   clear_error();
   if (!out) return fail(SLSF_E_INVALID, "output handle pointer is null");

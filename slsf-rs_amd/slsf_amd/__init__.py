@@ -196,6 +196,7 @@ _SIGS = {
     "slsf_server_poll": (ctypes.c_int, [ctypes.c_void_p, _P(JobResult), ctypes.c_int32, _i32p]),
     "slsf_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ServerStats)]),
     "slsf_server_slot_bytes": (ctypes.c_int64, [ctypes.c_int32]),
+    "slsf_device_memory": (ctypes.c_int, [_i64p, _i64p]),
     "slsf_run_theta_simulation": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, ctypes.c_double, _dp, _dp,
@@ -463,13 +464,23 @@ def jackknife(loo, f_all: float):
 
 
 # ---- serving -----------------------------------------------------------------------
-class Server:
-    """Continuous-batching job server (slsf_server_*)."""
+def device_memory() -> tuple[int, int]:
+    """(free, total) bytes of the current device."""
+    f, t = ctypes.c_int64(), ctypes.c_int64()
+    _check(_lib.slsf_device_memory(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
-    def __init__(self, device_budget_bytes: int, max_slots_per_size: int = 256,
+
+class Server:
+    """Continuous-batching job server (slsf_server_*).  Without a budget it
+    admits against 90% of the device's free memory."""
+
+    def __init__(self, device_budget_bytes: int | None = None, max_slots_per_size: int = 256,
                  chunk_sweeps: int = 100):
         h = ctypes.c_void_p()
         self._h = None
+        if device_budget_bytes is None:
+            device_budget_bytes = int(0.9 * device_memory()[0])
         _check(_lib.slsf_server_create(int(device_budget_bytes), max_slots_per_size,
                                        chunk_sweeps, ctypes.byref(h)))
         self._h = h
