@@ -169,19 +169,24 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   uint32_t nacc_wave = 0;  // wave-uniform: accepted proposals of this wave
   long long ncluster = 0;  // thread 0: cluster sites
 
+  // LDS slot of site (x, y, z) is (x + L y + L^2 z) / 2 = (x + L y) / 2 + HL z
+  // (L^2 z is even): a per-thread base plus a compile-time plane offset
+  constexpr int HL = L * L / 2;
+
   // write (cos, sin) of this thread's sites of colour `col` to LDS slot idx/2
   auto exchange = [&](int col) __attribute__((always_inline)) {
     asm volatile("" : "+v"(x), "+v"(y));
     const int z1 = (col + x + y) & 1;
     if (!own) return;
 #pragma unroll
-    for (int j = 0; j < CPT; ++j)
+    for (int j = 0; j < CPT; ++j) {
+      const int b = ((x + L * (y + j * S::YS)) >> 1) + HL * z1;
 #pragma unroll
       for (int k = 0; k < L / 2; ++k) {
-        const int z = 2 * k + z1;
         const float tv = sel(z1, th[j][2 * k + 1], th[j][2 * k]);
-        xbuf[(x + L * (y + j * S::YS + L * z)) >> 1] = make_float2(cos_t(tv), sin_t(tv));
+        xbuf[b + 2 * HL * k] = make_float2(cos_t(tv), sin_t(tv));
       }
+    }
   };
 
   // Metropolis update of this thread's colour-p sites
@@ -191,25 +196,34 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const int z0 = (p + x + y) & 1;
     if (!own) return;
     const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+    const float4* jz0 = jt + z0;
     Words w{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
     const int y = y0 + j * S::YS;
     const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+    // slots of the first site (z = z0) of the column and of its neighbours
+    const int s0 = HL * z0;
+    const int bxp = ((xp + L * y) >> 1) + s0, bxm = ((xm + L * y) >> 1) + s0;
+    const int byp = ((x + L * yp) >> 1) + s0, bym = ((x + L * ym) >> 1) + s0;
+    const int bz = ((x + L * y) >> 1) + s0;
 #pragma unroll
     for (int k = 0; k < L / 2; ++k) {
-      const int z = 2 * k + z0;
       const int ka = 2 * k, kb = 2 * k + 1;
       const float t0 = sel(z0, th[j][kb], th[j][ka]);
-      const int zn = (z + 1 == L) ? 0 : z + 1, zq = (z == 0) ? L - 1 : z - 1;
-      const int row = L * (y + L * z);
-      const float2 nxp = xbuf[(xp + row) >> 1];
-      const float2 nxm = xbuf[(xm + row) >> 1];
-      const float2 nyp = xbuf[(x + L * (yp + L * z)) >> 1];
-      const float2 nym = xbuf[(x + L * (ym + L * z)) >> 1];
-      const float2 nzp = xbuf[(x + L * (y + L * zn)) >> 1];
-      const float2 nzm = xbuf[(x + L * (y + L * zq)) >> 1];
-      const float4 jj = jt[z];
+      const int zo = 2 * HL * k;  // plane offset of z = 2k + z0
+      // z neighbours, periodic: z = L - 1 (last k, z0 = 1) wraps up to 0,
+      // z = 0 (first k, z0 = 0) wraps down to L - 1
+      const int dzp = (k == L / 2 - 1) ? (z0 ? -HL * (L - 1) : HL) : HL;
+      const int dzm = (k == 0) ? (z0 ? -HL : HL * (L - 1)) : -HL;
+      const float2 nxp = xbuf[bxp + zo];
+      const float2 nxm = xbuf[bxm + zo];
+      const float2 nyp = xbuf[byp + zo];
+      const float2 nym = xbuf[bym + zo];
+      const float2 nzp = xbuf[bz + zo + dzp];
+      const float2 nzm = xbuf[bz + zo + dzm];
+      const float4 jj = jz0[2 * k];
+      const int row = L * (y + L * (2 * k + z0));
       const float hx = jj.x * (nxp.x + nxm.x + nyp.x + nym.x) + jj.y * nzp.x + jj.z * nzm.x;
       const float hy = jj.x * (nxp.y + nxm.y + nyp.y + nym.y) + jj.y * nzp.y + jj.z * nzm.y;
       float up, ua;
@@ -246,28 +260,32 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     if (own) {
       const int z0 = (x + y) & 1;  // colour-0 sites of the column: z = 2k + z0
       const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+      const float4* jz0 = jt + z0;
       const int y0 = y;
 #pragma unroll
       for (int j = 0; j < CPT; ++j) {
       const int y = y0 + j * S::YS;
       const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+      const int s0 = HL * z0;
+      const int bxp = ((xp + L * y) >> 1) + s0, bxm = ((xm + L * y) >> 1) + s0;
+      const int byp = ((x + L * yp) >> 1) + s0, bym = ((x + L * ym) >> 1) + s0;
+      const int bz = ((x + L * y) >> 1) + s0;
 #pragma unroll
       for (int k = 0; k < L / 2; ++k) {
-        const int z = 2 * k + z0;
         const int ka = 2 * k, kb = 2 * k + 1;
-        const int kn = (2 * k + 2) % L, kp = (2 * k - 1 + L) % L;
         const float t0 = sel(z0, th[j][kb], th[j][ka]);
         const float cc = cos_t(t0), ss = sin_t(t0);
-        const int row = L * (y + L * z);
-        const float2 nxp = xbuf[(xp + row) >> 1];
-        const float2 nxm = xbuf[(xm + row) >> 1];
-        const float2 nyp = xbuf[(x + L * (yp + L * z)) >> 1];
-        const float2 nym = xbuf[(x + L * (ym + L * z)) >> 1];
-        // colour-1 neighbours in z: read back from the exchange
-        const int zn = z0 ? kn : kb, zq = z0 ? ka : kp;
-        const float2 nzp = xbuf[(x + L * (y + L * zn)) >> 1];
-        const float2 nzm = xbuf[(x + L * (y + L * zq)) >> 1];
-        const float4 jj = jt[z];
+        const int zo = 2 * HL * k;
+        const float2 nxp = xbuf[bxp + zo];
+        const float2 nxm = xbuf[bxm + zo];
+        const float2 nyp = xbuf[byp + zo];
+        const float2 nym = xbuf[bym + zo];
+        // colour-1 neighbours in z, read back from the exchange (periodic)
+        const int dzp = (k == L / 2 - 1) ? (z0 ? -HL * (L - 1) : HL) : HL;
+        const int dzm = (k == 0) ? (z0 ? -HL : HL * (L - 1)) : -HL;
+        const float2 nzp = xbuf[bz + zo + dzp];
+        const float2 nzm = xbuf[bz + zo + dzm];
+        const float4 jj = jz0[2 * k];
         // bond (a -> a + e): cos = ca cb + sa sb, sin = sa cb - ca sb
         e_c[0] += jj.x * ((cc * nxp.x + ss * nxp.y) + (nxm.x * cc + nxm.y * ss));
         e_s[0] += jj.x * ((ss * nxp.x - cc * nxp.y) + (nxm.y * cc - nxm.x * ss));
