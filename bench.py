@@ -222,7 +222,7 @@ def bench_c3(args, slsf):
          metropolis_only_rate=metro_rate, wolff_only_rate=wolff_rate,
          acceptance=st["acceptance_mean"], mean_cluster_size=st["mean_cluster_size"],
          roofline=roofline, cpu_baseline=cpu,
-         kernel={"name": "k_resident<32, paired>", "launches": st["launches"],
+         kernel={"name": "k_resident<32, 1, paired>", "launches": st["launches"],
                  "avg_launch_ms": avg_launch_ms, "sweeps_per_launch": spl,
                  "algorithmic_bytes_per_launch": alg_bytes_launch})
 

@@ -3,11 +3,11 @@
 //
 // Mapping (cubic L in {8, 12, 16, 24, 32}).  Thread t owns the z-column
 // (x, y) = (t % RW, t / RW), RW = next power of two >= L (lanes x >= L idle).
-// A site's state is the unit vector (cos theta, sin theta) in two FP32
-// registers; a Metropolis proposal theta -> theta + delta is the rotation by
-// delta, so one hardware sincos (of delta) is the only transcendental a site
-// update needs besides the Boltzmann factor.  Couplings J_xy[z], J_z[z],
-// J_z[z-1] sit in an LDS table.
+// A site's state is its angle in turns (revolutions, [0, 1)) in one FP32
+// register; the hardware v_sin/v_cos take turns, so a Metropolis update costs
+// the sin/cos of the old and the proposed angle and one exp2 for the
+// Boltzmann factor, with the neighbours' (cos, sin) read from LDS.  Couplings
+// J_xy[z], J_z[z], J_z[z-1] sit in an LDS table.
 //
 // Metropolis, colour p (src/updates.rs:440-486): all threads write the
 // (cos, sin) of their opposite-colour sites into LDS slot idx/2 (every pair of
@@ -20,9 +20,9 @@
 // its keyed uniforms; the decisions become LDS bitmaps, one 32-bit word per
 // (y, z) row (wave ballots).  The cluster is the seed's component of that bond
 // graph -- the same set the reference's DFS builds from the same uniforms --
-// found by a monotone flood fill over rows (x-runs closed inside the word, y
-// and z by neighbouring words) until a round changes nothing.  Members are
-// reflected in registers.
+// found by a monotone flood fill over rows (x-runs closed inside the word by
+// a doubling closure, y and z by neighbouring words) until a round changes
+// nothing.  Members are reflected in registers.
 //
 // Measurement (src/observables.rs:80-154): after the colour-1 exchange of the
 // next sweep, each colour-0 site sums its six bonds (each bond has exactly one
@@ -74,13 +74,29 @@ struct ResArgs {
   int32_t bin_cap, nsweeps;
 };
 
+
 template <int L>
-SLSF_D uint32_t rotl1(uint32_t w) {
-  return ((w << 1) | (w >> (L - 1))) & RShape<L, 1>::MASK;
+SLSF_D uint32_t rotl_s(uint32_t w, int s) {
+  return ((w << s) | (w >> (L - s))) & RShape<L, 1>::MASK;
 }
 template <int L>
-SLSF_D uint32_t rotr1(uint32_t w) {
-  return (w >> 1) | ((w & 1u) << (L - 1));
+SLSF_D uint32_t rotr_s(uint32_t w, int s) {
+  return ((w >> s) | (w << (L - s))) & RShape<L, 1>::MASK;
+}
+// Closure of the marked sites n of a periodic row through its active x-bonds
+// b (bit i: sites i and i+1 mod L bonded) by doubling: P_s(i) = sites i and
+// i+s joined by a chain of bonds, P_2s = P_s & rotr(P_s, s); with s = 1, 2,
+// 4, ... < L every distance d < L is a sum of increasing steps, so every site
+// of a bonded run that holds a mark ends up marked.
+template <int L>
+SLSF_D uint32_t row_closure(uint32_t n, uint32_t b) {
+  uint32_t p = b;
+#pragma unroll
+  for (int s = 1; s < L; s *= 2) {
+    n |= rotl_s<L>(n & p, s) | (rotr_s<L>(n, s) & p);
+    p &= rotr_s<L>(p, s);
+  }
+  return n;
 }
 
 // An empty asm makes a value opaque to the optimizer.  sel() makes both
@@ -384,12 +400,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         const uint32_t v = vis[r];
         uint32_t n = v | (vis[rowym] & bym[rowym]) | (vis[rowyp] & bym[r]) |
                      (vis[rowzm] & bzm[rowzm]) | (vis[rowzp] & bzm[r]);
-        const uint32_t bx = bxm[r];
-        for (int it = 0; it < L; ++it) {
-          const uint32_t m = n | rotl1<L>(n & bx) | (rotr1<L>(n) & bx);
-          if (m == n) break;
-          n = m;
-        }
+        n = row_closure<L>(n, bxm[r]);
         if (n != v) {
           vis[r] = n;
           changed = true;
