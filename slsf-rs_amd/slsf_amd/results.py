@@ -1,0 +1,229 @@
+"""Job runner and result files of the GPU path, in the reference's formats.
+
+* ``run_job`` plays the carlo-mc ``Runner`` for one rank (src/results.rs:187-239):
+  the rank's tasks of a TOML job (``idx % world == rank``, src/model.rs:62-69)
+  run as one FP32 batch on the GPU -- thermalization, then ``sweeps`` measured
+  sweeps binned ``binsize`` at a time on the device -- and each task gets the
+  estimates ``finalize_theta_estimates`` produces (src/model.rs:906-934): every
+  measured series as a binned mean with its error, plus the evaluables
+  Magnetization, Chi, SpecificHeat, RhoXY, RhoZ (src/model.rs:857-888) and the
+  Binder ratio, with jackknife errors.
+* ``write_job_result`` / ``read_job_result`` / ``merge_job_results`` follow
+  ``ThetaJobResult`` (src/model.rs:71-106) and src/results.rs:14-36,241-272:
+  ``{job_name}.rank{rank}.results.json``, pretty-printed, merge sorted by task
+  name with the reference's messages.
+* ``dataframe_from_result`` / ``write_dataframe`` restate src/result_tools.rs:47-
+  112,160-251 (columns, ``{:.16}`` floats, quoting of delimited fields).
+
+An estimate is written as ``{"mean", "error", "internal_bin_len"}``, the
+fields of carlo-mc's ``Estimate`` the reference itself reads
+(src/results.rs:74-83; src/result_tools.rs:200-207).  Series the batched kernels
+do not measure -- the correlation functions and the ``_ll_*`` timers -- are
+absent from ``observables``.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from typing import Sequence
+
+import numpy as np
+
+from . import (EVALS, FP32, SERIES, Batch, Job, Replica, SlsfError, evaluate)
+
+# fields of ThetaTask (src/model.rs:20-47), in declaration order
+TASK_FIELDS = ["name", "l", "l_x", "l_y", "l_z", "temperature", "j_xy", "delta_j_xy",
+               "j_z_mean", "delta_j_z", "disorder_seed", "seed", "sample", "sweeps",
+               "thermalization", "binsize", "proposal_width", "wolff_steps",
+               "correlation_rmax", "correlation_rmax_xy", "correlation_rmax_z",
+               "correlation_interval", "j_xy_array", "j_z_array"]
+
+
+def _task_json(t: dict) -> dict:
+    out = {}
+    for k in TASK_FIELDS:
+        v = t[k]
+        if isinstance(v, np.ndarray):
+            v = [float(x) for x in v]
+        elif isinstance(v, (np.floating, float)):
+            v = float(v)
+        elif isinstance(v, (np.integer, int)):
+            v = int(v)
+        out[k] = v
+    return out
+
+
+def _binned(bins: np.ndarray, bin_len: int) -> dict:
+    """BinnedEstimate semantics (src/tests.rs:436-446): mean of the bins and
+    std(bins) / sqrt(nbins)."""
+    n = len(bins)
+    mean = float(np.mean(bins)) if n else float("nan")
+    err = float(np.std(bins, ddof=1) / math.sqrt(n)) if n > 1 else float("nan")
+    return {"mean": mean, "error": err, "internal_bin_len": int(bin_len)}
+
+
+def run_job(job: Job, rank: int = 0, world_size: int = 1, precision: int = FP32,
+            sweeps_per_launch: int = 0, use_graph: bool = False) -> dict:
+    """Runs this rank's tasks of `job` on the GPU and returns a ThetaJobResult."""
+    if world_size < 1 or not 0 <= rank < world_size:
+        raise SlsfError(1, "rank must be below world_size")
+    idx = job.selected_tasks(rank, world_size)
+    tasks = [job.task(i) for i in idx]
+    result = {"job_name": job.name, "rank": rank, "world_size": world_size, "tasks": []}
+    if not tasks:
+        return result
+    therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
+    reps = [Replica(t["l_x"], t["l_y"], t["l_z"], t["temperature"], t["j_xy_array"],
+                    t["j_z_array"], int(t["seed"]), t["proposal_width"], int(t["wolff_steps"]))
+            for t in tasks]
+    b = Batch(reps, precision)
+    try:
+        b.init_angles()
+        if therm:
+            b.run(therm, 0, 1, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
+        if sweeps:
+            b.run(0, sweeps, binsize, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
+        acceptance = b.acceptance()
+        for k, (i, t) in enumerate(zip(idx, tasks)):
+            bins = b.all_bins(k) if sweeps else np.zeros((len(SERIES), 0))
+            obs = {name: _binned(bins[s], binsize) for s, name in enumerate(SERIES)}
+            if bins.shape[1] > 0:
+                ev = evaluate(bins, float(t["l_x"] * t["l_y"] * t["l_z"]), 1.0 / t["temperature"])
+                for name in EVALS:
+                    if name in obs:
+                        continue
+                    mean, err = ev[name]
+                    obs[name] = {"mean": float(mean), "error": float(err),
+                                 "internal_bin_len": binsize}
+            entry = {"task": _task_json(t)}
+            if i != 0:
+                entry["task_index"] = int(i)  # serialised only when nonzero
+            entry.update({"observables": dict(sorted(obs.items())),
+                          "acceptance": float(acceptance[k]), "measurements": sweeps})
+            result["tasks"].append(entry)
+    finally:
+        b.close()
+    return result
+
+
+def result_path(job_name: str, rank: int, output_dir: str) -> str:
+    return os.path.join(output_dir, f"{job_name}.rank{rank}.results.json")
+
+
+def _finite(x):
+    """JSON has no NaN/Inf: serde_json writes them as null."""
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    if isinstance(x, dict):
+        return {k: _finite(v) for k, v in x.items()}
+    if isinstance(x, list):
+        return [_finite(v) for v in x]
+    return x
+
+
+def write_job_result(result: dict, output_dir: str) -> str:
+    os.makedirs(output_dir, exist_ok=True)
+    return write_job_result_to_path(result, result_path(result["job_name"], result["rank"],
+                                                        output_dir))
+
+
+def write_job_result_to_path(result: dict, path: str) -> str:
+    parent = os.path.dirname(path)
+    if parent:
+        os.makedirs(parent, exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(_finite(result), f, indent=2)
+    return path
+
+
+def read_job_result(path: str) -> dict:
+    with open(path) as f:
+        return json.load(f)
+
+
+def merge_job_results(results: Sequence[dict]) -> dict:
+    """src/results.rs:246-272."""
+    if not results:
+        raise ValueError("at least one rank result is required for merge")
+    first = results[0]
+    tasks = list(first["tasks"])
+    for r in results[1:]:
+        if r["job_name"] != first["job_name"]:
+            raise ValueError("cannot merge results from different jobs")
+        if r["world_size"] != first["world_size"]:
+            raise ValueError("cannot merge results with different world sizes")
+        tasks.extend(r["tasks"])
+    tasks = sorted(tasks, key=lambda t: t["task"]["name"])
+    for k, t in enumerate(tasks):
+        if t.get("task_index", 0) == 0:
+            t = dict(t)
+            if k != 0:
+                t["task_index"] = k
+            tasks[k] = t
+    return {"job_name": first["job_name"], "rank": 0, "world_size": first["world_size"],
+            "tasks": tasks}
+
+
+def format_float(v) -> str:
+    """src/result_tools.rs:241-251 ({:.16} for finite values)."""
+    if v is None:
+        return "NaN"
+    v = float(v)
+    if math.isfinite(v):
+        return f"{v:.16f}"
+    if math.isnan(v):
+        return "NaN"
+    return "-Inf" if v < 0 else "Inf"
+
+
+def dataframe_from_result(result: dict) -> tuple[list[str], list[dict]]:
+    """Columns and rows of src/result_tools.rs:54-96,160-205."""
+    names = sorted({n for t in result["tasks"] for n in t["observables"]})
+    columns = ["job_name", "rank", "world_size", "task_index", "task_name", "L", "Lx", "Ly",
+               "Lz", "T", "Jxy", "DeltaJxy", "Jz", "DeltaJz", "sample", "seed",
+               "disorder_seed", "sweeps", "thermalization", "binsize", "measurements",
+               "acceptance"]
+    for n in names:
+        columns += [n, f"{n}_error", f"{n}_measurement"]
+    rows = []
+    for tr in result["tasks"]:
+        t = tr["task"]
+        row = {"job_name": result["job_name"], "rank": str(result["rank"]),
+               "world_size": str(result["world_size"]),
+               "task_index": str(tr.get("task_index", 0)), "task_name": t["name"],
+               "L": str(t["l"]), "Lx": str(t["l_x"]), "Ly": str(t["l_y"]), "Lz": str(t["l_z"]),
+               "T": format_float(t["temperature"]), "Jxy": format_float(t["j_xy"]),
+               "DeltaJxy": format_float(t["delta_j_xy"]), "Jz": format_float(t["j_z_mean"]),
+               "DeltaJz": format_float(t["delta_j_z"]), "sample": str(t["sample"]),
+               "seed": str(t["seed"]), "disorder_seed": str(t["disorder_seed"]),
+               "sweeps": str(t["sweeps"]), "thermalization": str(t["thermalization"]),
+               "binsize": str(t["binsize"]), "measurements": str(tr["measurements"]),
+               "acceptance": format_float(tr["acceptance"])}
+        for n, e in tr["observables"].items():
+            m, s = format_float(e["mean"]), format_float(e["error"])
+            row[n] = m
+            row[f"{n}_error"] = s
+            row[f"{n}_measurement"] = f"{m} +/- {s}"
+        rows.append(row)
+    return columns, rows
+
+
+def _escape(field: str, delim: str) -> str:
+    if delim in field or "\n" in field or "\r" in field or '"' in field:
+        return '"' + field.replace('"', '""') + '"'
+    return field
+
+
+def write_dataframe(result_json: str, output_path: str) -> str:
+    """Comma-separated for a .csv path, tab-separated otherwise."""
+    columns, rows = dataframe_from_result(read_job_result(result_json))
+    delim = "," if output_path.endswith(".csv") else "\t"
+    parent = os.path.dirname(output_path)
+    if parent:
+        os.makedirs(parent, exist_ok=True)
+    with open(output_path, "w") as f:
+        f.write(delim.join(_escape(c, delim) for c in columns) + "\n")
+        for r in rows:
+            f.write(delim.join(_escape(r.get(c, ""), delim) for c in columns) + "\n")
+    return output_path

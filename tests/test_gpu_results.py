@@ -1,0 +1,57 @@
+"""GPU job runner (slsf_amd.results.run_job): a TOML job's tasks run as one
+batch; each task's estimates equal those of the same replica run alone (keyed
+uniforms), the JSON and dataframe files are written, and ranks split the tasks
+as the reference does (idx % world == rank) and merge back to the full job."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+slsf = pytest.importorskip("slsf_amd")
+from slsf_amd import results as R  # noqa: E402
+
+JOB = ('name = "gpu_results_unit"\n[model]\nL=[8,12]\nT=[1.1,1.6]\nsamples=2\nbase_seed=3\n'
+       'disorder="uniform"\nj_xy_min=0.6\nj_xy_max=1.8\nj_z=1.0\n'
+       '[run]\nsweeps=60\nthermalization=20\nbinsize=10\n')
+
+
+def test_run_job_files_and_alone_equivalence(tmp_path):
+    job = slsf.Job(JOB)
+    res = R.run_job(job)
+    assert res["job_name"] == "gpu_results_unit" and len(res["tasks"]) == 8
+    for k, tr in enumerate(res["tasks"]):
+        obs = tr["observables"]
+        for name in ("Energy", "MagnetizationSquared", "_helicity_cos_x", "RhoXY", "RhoZ",
+                     "Chi", "SpecificHeat", "Magnetization", "Binder"):
+            assert name in obs and np.isfinite(obs[name]["mean"]), name
+        assert obs["Energy"]["internal_bin_len"] == 10 and tr["measurements"] == 60
+        assert 0 < tr["acceptance"] < 1
+        assert tr.get("task_index", 0) == k
+    # task 5 alone: identical bins, so identical estimates
+    t = job.task(5)
+    b = slsf.Batch([job.replica(5)], slsf.FP32)
+    b.init_angles()
+    b.run(20, 0, 1)
+    b.run(0, 60, 10)
+    bins = b.all_bins(0)
+    e = res["tasks"][5]["observables"]["Energy"]
+    assert e["mean"] == float(np.mean(bins[0]))
+    assert res["tasks"][5]["task"]["name"] == t["name"]
+    p = R.write_job_result(res, str(tmp_path))
+    frame = R.write_dataframe(p, str(tmp_path / "frame.tsv"))
+    lines = open(frame).read().splitlines()
+    assert len(lines) == 9 and "RhoXY_measurement" in lines[0].split("\t")
+
+
+def test_ranks_split_and_merge():
+    job = slsf.Job(JOB)
+    parts = [R.run_job(job, rank=r, world_size=3) for r in range(3)]
+    assert sorted(len(p["tasks"]) for p in parts) == [2, 3, 3]
+    merged = R.merge_job_results(parts)
+    full = R.run_job(job)
+    assert [t["task"]["name"] for t in merged["tasks"]] == sorted(
+        t["task"]["name"] for t in full["tasks"])
+    by_name = {t["task"]["name"]: t for t in full["tasks"]}
+    for t in merged["tasks"]:
+        assert t["observables"]["Energy"] == by_name[t["task"]["name"]]["observables"]["Energy"]
+    with pytest.raises(slsf.SlsfError):
+        R.run_job(job, rank=3, world_size=3)
