@@ -33,10 +33,12 @@ def wrapped_diff(a, b):
     return np.minimum(d, 2 * math.pi - d)
 
 
-@pytest.mark.parametrize("L", [8, 16, 32])
+@pytest.mark.parametrize("L", [8, 16, 32, 48])
 def test_replay_stream_metropolis_decisions(L):
     """Metropolis only (wolff_steps=0): decision mismatches per 1e6 updates
-    between the FP32 kernel and the FP64 replay kernel on the same uniforms."""
+    between the FP32 kernels and the FP64 replay kernel on the same uniforms
+    (L <= 32: the resident kernel's replay stream; L = 48: the FP32 HBM
+    kernels, which always draw the per-site stream)."""
     n = 8
     reps = reps_for(L, n, wolff=0)
     b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
@@ -59,7 +61,7 @@ def test_replay_stream_metropolis_decisions(L):
     assert maxd < 2e-5
 
 
-@pytest.mark.parametrize("L", [8, 16])
+@pytest.mark.parametrize("L", [8, 16, 48])
 def test_replay_stream_full_sweep(L):
     """Metropolis + Wolff: most replicas reproduce the FP64 sweep to FP32
     rounding (a cluster flips differently only if a bond's uniform falls within

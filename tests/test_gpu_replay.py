@@ -35,6 +35,9 @@ CASES = [
     (2, 4, 6, 1.0, 99, np.pi),
     (8, 8, 8, 1.3, 12345, np.pi),
     (16, 16, 16, 1.3, 1, np.pi),
+    # several 16^3 union-find tiles, clipped tiles and wrapped faces, large clusters
+    (20, 18, 34, 1.1, 7, np.pi),
+    (36, 4, 2, 0.8, 21, np.pi),
 ]
 
 
