@@ -310,8 +310,7 @@ int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t prec
   int rc = SLSF_OK;
   rc = rc ? rc : bp->alloc((RepDev**)&g.reps, (size_t)n);
   rc = rc ? rc : bp->alloc((char**)&g.theta, rs * (size_t)site_off);
-  rc = rc ? rc : bp->alloc((char**)&g.cth, rs * (size_t)site_off);
-  rc = rc ? rc : bp->alloc((char**)&g.sth, rs * (size_t)site_off);
+  rc = rc ? rc : bp->alloc((char**)&g.cs, 2 * rs * (size_t)site_off);
   rc = rc ? rc : bp->alloc((char**)&g.jxy, rs * (size_t)jz_off);
   rc = rc ? rc : bp->alloc((char**)&g.jz, rs * (size_t)jz_off);
   rc = rc ? rc : bp->alloc(&g.bondmask, (size_t)site_off);

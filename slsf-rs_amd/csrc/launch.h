@@ -15,8 +15,7 @@ struct GlobalState {
   int64_t max_volume = 0;   // largest replica volume
   int64_t total_sites = 0;  // sum of padded volumes
   void* theta = nullptr;    // R[total_sites]
-  void* cth = nullptr;
-  void* sth = nullptr;
+  void* cs = nullptr;       // (cos, sin) pairs of theta: R2[total_sites]
   void* jxy = nullptr;      // R[total_layers]
   void* jz = nullptr;
   int max_lz = 0;           // largest replica extents (grid sizes)

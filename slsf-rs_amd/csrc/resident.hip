@@ -62,8 +62,7 @@ struct ResArgs {
   RepDev* reps;
   const int32_t* index;  // batch replica index per workgroup
   float* theta;
-  float* cth;
-  float* sth;
+  float2* cs;
   const float* jxy;
   const float* jz;
   double* accum;
@@ -463,8 +462,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       for (int z = 0; z < L; ++z) {
         const int64_t g = base + x + L * (y + j * S::YS + L * z);
         a.theta[g] = th[j][z];
-        a.cth[g] = cos_t(th[j][z]);
-        a.sth[g] = sin_t(th[j][z]);
+        a.cs[g] = make_float2(cos_t(th[j][z]), sin_t(th[j][z]));
       }
   }
   double tallies[2] = {(t & 63) == 0 ? (double)nacc_wave : 0.0, (double)ncluster};
@@ -558,8 +556,7 @@ static ResArgs make_args(const ResidentGroup& grp, const GlobalState& g,
   a.reps = g.reps;
   a.index = grp.d_index;
   a.theta = (float*)g.theta;
-  a.cth = (float*)g.cth;
-  a.sth = (float*)g.sth;
+  a.cs = (float2*)g.cs;
   a.jxy = (const float*)g.jxy;
   a.jz = (const float*)g.jz;
   a.accum = ra.accum;
@@ -673,11 +670,10 @@ int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
                       int trial_sweeps, std::vector<TuneRecord>& records) {
   records.clear();
   const size_t n = (size_t)g.total_sites;
-  float *th = nullptr, *c = nullptr, *sn = nullptr;
+  float *th = nullptr, *c = nullptr;
   RepDev* rd = nullptr;
   RES_TRY(hipMalloc((void**)&th, n * sizeof(float)), "hipMalloc failed");
-  RES_TRY(hipMalloc((void**)&c, n * sizeof(float)), "hipMalloc failed");
-  RES_TRY(hipMalloc((void**)&sn, n * sizeof(float)), "hipMalloc failed");
+  RES_TRY(hipMalloc((void**)&c, n * 2 * sizeof(float)), "hipMalloc failed");
   RES_TRY(hipMalloc((void**)&rd, (size_t)g.nrep * sizeof(RepDev)), "hipMalloc failed");
   auto copy_all = [&](bool save) -> hipError_t {
     hipError_t e = hipSuccess;
@@ -685,10 +681,10 @@ int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
       if (e == hipSuccess) e = hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, s);
     };
     if (save) {
-      cp(th, g.theta, n * 4); cp(c, g.cth, n * 4); cp(sn, g.sth, n * 4);
+      cp(th, g.theta, n * 4); cp(c, g.cs, n * 8);
       cp(rd, g.reps, g.nrep * sizeof(RepDev));
     } else {
-      cp(g.theta, th, n * 4); cp(g.cth, c, n * 4); cp(g.sth, sn, n * 4);
+      cp(g.theta, th, n * 4); cp(g.cs, c, n * 8);
       cp(g.reps, rd, g.nrep * sizeof(RepDev));
     }
     return e == hipSuccess ? hipStreamSynchronize(s) : e;
@@ -745,7 +741,7 @@ int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
   if (!rc && copy_all(false) != hipSuccess) rc = fail(SLSF_E_DEVICE, "autotune restore failed");
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  (void)hipFree(th); (void)hipFree(c); (void)hipFree(sn); (void)hipFree(rd);
+  (void)hipFree(th); (void)hipFree(c); (void)hipFree(rd);
   if (acc) (void)hipFree(acc);
   if (cs) (void)hipFree(cs);
   return rc;

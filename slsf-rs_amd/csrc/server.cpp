@@ -151,8 +151,7 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   int rc = SLSF_OK;
   rc = rc ? rc : p->alloc((RepDev**)&g.reps, (size_t)cap);
   rc = rc ? rc : p->alloc((float**)&g.theta, n);
-  rc = rc ? rc : p->alloc((float**)&g.cth, n);
-  rc = rc ? rc : p->alloc((float**)&g.sth, n);
+  rc = rc ? rc : p->alloc((float2**)&g.cs, n);
   rc = rc ? rc : p->alloc((float**)&g.jxy, (size_t)cap * L);
   rc = rc ? rc : p->alloc((float**)&g.jz, (size_t)cap * L);
   rc = rc ? rc : p->alloc(&g.acc, (size_t)cap * TALLY_SLOTS);
