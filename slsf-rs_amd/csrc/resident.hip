@@ -21,8 +21,9 @@
 // (y, z) row (wave ballots).  The cluster is the seed's component of that bond
 // graph -- the same set the reference's DFS builds from the same uniforms --
 // found by a monotone flood fill over rows (x-runs closed inside the word by
-// a doubling closure, y and z by neighbouring words) until a round changes
-// nothing.  Members are reflected in registers.
+// a doubling closure, y and z through the words 1, 2 and 4 rows away along
+// straight bonded lines) until a round changes nothing.  Members are reflected
+// in registers.
 //
 // Measurement (src/observables.rs:80-154): after the colour-1 exchange of the
 // next sweep, each colour-0 site sums its six bonds (each bond has exactly one
@@ -56,6 +57,7 @@ struct RShape {
   static_assert(NT % 64 == 0, "workgroup must be whole waves");
   static_assert((L / 2) % 2 == 0, "pairs of same-colour sites per column");
   static_assert(YS % 2 == 0, "a thread's columns share their colour parity");
+  static_assert(4 * ROWS <= V, "the fill's four path-mask arrays fit in the exchange buffer");
 };
 
 struct ResArgs {
@@ -147,7 +149,7 @@ SLSF_D void wg_sum(double (&v)[N], double* red) {
 template <int L, int CPT, bool PAIRED>
 __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   using S = RShape<L, CPT>;
-  __shared__ float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V]
+  __shared__ float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V], then path masks
   __shared__ uint32_t bxm[S::ROWS], bym[S::ROWS], bzm[S::ROWS], vis[S::ROWS];
   __shared__ float4 jt[L];           // {J_xy[z], J_z[z], J_z[z-1], 0}
   __shared__ double red[S::NW * 9];
@@ -388,6 +390,27 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     }
     for (int r = t; r < S::ROWS; r += S::NT) vis[r] = (r == seed / L) ? (1u << (seed % L)) : 0u;
     __syncthreads();
+    // Straight-line path masks, in the exchange buffer (free during the fill):
+    // bit x of py2[r] says the y-bonds of column x from row r to row r + 2 (same
+    // z) are both active, py4 the same over four rows; pz2/pz4 along z.  A round
+    // then moves a mark up to four rows in a straight line, which cuts the
+    // number of rounds without changing the fixed point.
+    uint32_t* py2 = reinterpret_cast<uint32_t*>(xbuf);
+    uint32_t* pz2 = py2 + S::ROWS;
+    uint32_t* py4 = pz2 + S::ROWS;
+    uint32_t* pz4 = py4 + S::ROWS;
+    for (int r = t; r < S::ROWS; r += S::NT) {
+      const int ry = r % L, rz = r / L;
+      py2[r] = bym[r] & bym[((ry + 1) % L) + L * rz];
+      pz2[r] = bzm[r] & bzm[ry + L * ((rz + 1) % L)];
+    }
+    __syncthreads();
+    for (int r = t; r < S::ROWS; r += S::NT) {
+      const int ry = r % L, rz = r / L;
+      py4[r] = py2[r] & py2[((ry + 2) % L) + L * rz];
+      pz4[r] = pz2[r] & pz2[ry + L * ((rz + 2) % L)];
+    }
+    __syncthreads();
     // flood fill to the fixed point; the flag of round r lives in slot r % 3,
     // which thread 0 clears only in round r + 2, after every thread read it
     for (int round = 0;; ++round) {
@@ -399,6 +422,14 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         const uint32_t v = vis[r];
         uint32_t n = v | (vis[rowym] & bym[rowym]) | (vis[rowyp] & bym[r]) |
                      (vis[rowzm] & bzm[rowzm]) | (vis[rowzp] & bzm[r]);
+        const int y2m = ((ry + L - 2) % L) + L * rz, y2p = ((ry + 2) % L) + L * rz;
+        const int z2m = ry + L * ((rz + L - 2) % L), z2p = ry + L * ((rz + 2) % L);
+        n |= (vis[y2m] & py2[y2m]) | (vis[y2p] & py2[r]) | (vis[z2m] & pz2[z2m]) |
+             (vis[z2p] & pz2[r]);
+        const int y4m = ((ry + L - 4) % L) + L * rz, y4p = ((ry + 4) % L) + L * rz;
+        const int z4m = ry + L * ((rz + L - 4) % L), z4p = ry + L * ((rz + 4) % L);
+        n |= (vis[y4m] & py4[y4m]) | (vis[y4p] & py4[r]) | (vis[z4m] & pz4[z4m]) |
+             (vis[z4p] & pz4[r]);
         n = row_closure<L>(n, bxm[r]);
         if (n != v) {
           vis[r] = n;
