@@ -358,34 +358,57 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const int wave = t >> 6, lane = t & 63;
     const float m2b = -2.0f * beta_l2e;
     const int xp = (x + 1 == L) ? 0 : x + 1;
-    // bond decisions read r from LDS only, so this loop stays rolled
+    // bond decisions read r from LDS only, so this loop stays rolled; it takes
+    // four sites (z .. z+3) of a column per step.  Production stream: the 12
+    // bond uniforms of the four sites are the words of three keyed calls (tags
+    // WBOND_QUAD + 0..2, index = the first site); replay stream: one call per
+    // site (tag WBOND, words 0..2), as in the FP64 path.
 #pragma unroll 1
-    for (int jz = 0; jz < CPT * L; ++jz) {
-      const int j = jz / L, z = jz % L;
+    for (int jq = 0; jq < CPT * L; jq += 4) {
+      const int j = jq / L, zq = jq % L;
       const int yj = y + j * S::YS;
       const int yp = (yj + 1 == L) ? 0 : yj + 1;
-      const int zn = (z + 1 == L) ? 0 : z + 1;
-      bool bx_ = false, by_ = false, bz_ = false;
-      if (own) {
-        const int i = x + L * (yj + L * z);
-        const float ri = rbuf[i];
-        const float rxp = rbuf[xp + L * (yj + L * z)];
-        const float ryp = rbuf[x + L * (yp + L * z)];
-        const float rzp = rbuf[x + L * (yj + L * zn)];
-        const float4 jj = jt[z];
-        const Words w = keyed(k0, k1, (uint32_t)i, wctr, TAG_WBOND);
-        const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
-        bx_ = (px > 0.0f) & (u24(w.w0) < 1.0f - exp2_hw(m2b * px));
-        by_ = (py > 0.0f) & (u24(w.w1) < 1.0f - exp2_hw(m2b * py));
-        bz_ = (pz > 0.0f) & (u24(w.w2) < 1.0f - exp2_hw(m2b * pz));
+      Words wq[3] = {};
+      if (PAIRED && own) {
+        const uint32_t i0 = (uint32_t)(x + L * (yj + L * zq));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) wq[c] = keyed(k0, k1, i0, wctr, TAG_WBOND_QUAD + c);
       }
-      const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
-      if (lane < S::RPW) {
-        const int row = wave * S::RPW + lane + j * S::YS + L * z;
-        const int sh = lane * S::RW;
-        bxm[row] = (uint32_t)(gx >> sh) & S::MASK;
-        bym[row] = (uint32_t)(gy >> sh) & S::MASK;
-        bzm[row] = (uint32_t)(gz >> sh) & S::MASK;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int z = zq + q;
+        const int zn = (z + 1 == L) ? 0 : z + 1;
+        bool bx_ = false, by_ = false, bz_ = false;
+        if (own) {
+          const int i = x + L * (yj + L * z);
+          const float ri = rbuf[i];
+          const float rxp = rbuf[xp + L * (yj + L * z)];
+          const float ryp = rbuf[x + L * (yp + L * z)];
+          const float rzp = rbuf[x + L * (yj + L * zn)];
+          const float4 jj = jt[z];
+          uint32_t u[3];
+          if (PAIRED) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) u[a] = word(wq[(3 * q + a) / 4], (3 * q + a) % 4);
+          } else {
+            const Words w = keyed(k0, k1, (uint32_t)i, wctr, TAG_WBOND);
+            u[0] = w.w0;
+            u[1] = w.w1;
+            u[2] = w.w2;
+          }
+          const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
+          bx_ = (px > 0.0f) & (u24(u[0]) < 1.0f - exp2_hw(m2b * px));
+          by_ = (py > 0.0f) & (u24(u[1]) < 1.0f - exp2_hw(m2b * py));
+          bz_ = (pz > 0.0f) & (u24(u[2]) < 1.0f - exp2_hw(m2b * pz));
+        }
+        const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
+        if (lane < S::RPW) {
+          const int row = wave * S::RPW + lane + j * S::YS + L * z;
+          const int sh = lane * S::RW;
+          bxm[row] = (uint32_t)(gx >> sh) & S::MASK;
+          bym[row] = (uint32_t)(gy >> sh) & S::MASK;
+          bzm[row] = (uint32_t)(gz >> sh) & S::MASK;
+        }
       }
     }
     for (int r = t; r < S::ROWS; r += S::NT) vis[r] = (r == seed / L) ? (1u << (seed % L)) : 0u;

@@ -19,6 +19,10 @@
 //      tag METRO_PAIR (FP32 resident kernel, production stream): index = the
 //        first of two same-colour sites z, z+2 of a column, counter = sweep
 //                                -> (proposal, accept) = (w0, w1) and (w2, w3)
+//      tags WBOND_QUAD + c, c = 0..2 (FP32 resident kernel, production stream):
+//        index = site (x, y, z) with z % 4 == 0, counter = cluster; the 12 words
+//        of the three calls, in order, are the +x, +y, +z bond uniforms of the
+//        sites z, z+1, z+2, z+3 of the column
 //    FP64 uniforms use 53 bits (two words), FP32 uniforms the top 24 bits of the
 //    first word of the pair; bond uniforms are w*2^-32 (FP64) or (w>>8)*2^-24.
 #pragma once
@@ -35,6 +39,7 @@ constexpr uint32_t TAG_METRO = 2;
 constexpr uint32_t TAG_WSEED = 3;
 constexpr uint32_t TAG_WBOND = 4;
 constexpr uint32_t TAG_METRO_PAIR = 5;
+constexpr uint32_t TAG_WBOND_QUAD = 6;  // and 7, 8: three calls per four sites
 
 constexpr double TWO_PI = 3.141592653589793 * 2.0;  // src/types.rs:6
 constexpr double PI = 3.141592653589793;
@@ -69,6 +74,10 @@ SLSF_HD Words philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
     k1 += 0xBB67AE85u;
   }
   return Words{c0, c1, c2, c3};
+}
+
+SLSF_HD uint32_t word(const Words& w, int k) {
+  return k == 0 ? w.w0 : k == 1 ? w.w1 : k == 2 ? w.w2 : w.w3;
 }
 
 SLSF_HD Words keyed(uint32_t k0, uint32_t k1, uint32_t index, uint32_t counter,
