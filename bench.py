@@ -152,9 +152,9 @@ def line(value, unit, steps, warmup, ms_per_step, workload, dtype, higher=True, 
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(spl, nrep):
-    """HBM bytes per launch of k_resident from the committed counter run, when
-    that run used the same launch shape; None otherwise."""
+def profile_record(spl, nrep):
+    """The committed counter run of k_resident (HBM bytes and VALU instructions
+    per launch), when that run used the same launch shape; None otherwise."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -162,7 +162,7 @@ def measured_traffic(spl, nrep):
         return None
     if t.get("sweeps_per_launch") != spl or t.get("replicas") != nrep:
         return None
-    return t["fetch_bytes"] + t["write_bytes"]
+    return t
 
 
 # ---------------------------------------------------------------------------
@@ -208,8 +208,24 @@ def bench_c3(args, slsf):
     avg_launch_ms = kernel_ms / launches
     achieved = (len(reps) * V * args.steps * bytes_per_update / (kernel_ms * 1e-3) / 1e9
                 if kernel_ms > 0 else value * bytes_per_update / 1e9)
+    prof = profile_record(spl, len(reps))
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(spl, len(reps))}
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": (prof["fetch_bytes"] + prof["write_bytes"]) if prof else None}
+    # the on-chip ceilings the resident kernel actually runs against, measured
+    # on this device: LDS bandwidth (32 B of LDS traffic per site update in the
+    # LDS form) and VALU issue (instructions per update from the counter run)
+    ceil = slsf.measure_ceilings()
+    kernel_rate = (len(reps) * V * args.steps / (kernel_ms * 1e-3)) if kernel_ms > 0 else value
+    lds = {"bytes_per_update": 32, "achieved_GBs": kernel_rate * 32 / 1e9,
+           "peak_measured_GBs": ceil["lds_bytes_per_s"] / 1e9,
+           "frac": kernel_rate * 32 / ceil["lds_bytes_per_s"]}
+    valu = None
+    if prof and prof.get("valu_per_update"):
+        issue = kernel_rate * prof["valu_per_update"] / 64.0
+        valu = {"instr_per_update": prof["valu_per_update"], "wave_instr_per_s": issue,
+                "fma_issue_measured": ceil["fma_wave_instr_per_s"],
+                "frac": issue / ceil["fma_wave_instr_per_s"]}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     line(value, "site-updates/s", args.steps, args.warmup, 1e3 * st["seconds"] / args.steps,
          {"workload": f"L=32 batched disorder average: {args.realizations} realizations x "
@@ -221,7 +237,8 @@ def bench_c3(args, slsf):
          batched_over_plain=(value / plain) if plain else None,
          metropolis_only_rate=metro_rate, wolff_only_rate=wolff_rate,
          acceptance=st["acceptance_mean"], mean_cluster_size=st["mean_cluster_size"],
-         roofline=roofline, cpu_baseline=cpu,
+         roofline=roofline, lds_roofline=lds, valu_issue=valu, ceilings=ceil,
+         cpu_baseline=cpu,
          kernel={"name": "k_resident<32, 1, paired>", "launches": st["launches"],
                  "avg_launch_ms": avg_launch_ms, "sweeps_per_launch": spl,
                  "algorithmic_bytes_per_launch": alg_bytes_launch})

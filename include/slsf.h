@@ -353,6 +353,23 @@ int64_t slsf_server_slot_bytes(int32_t l);
 int slsf_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
 /* ======================================================================== */
+/* Device ceilings (measurement helper; no counterpart in the reference)     */
+/* ======================================================================== */
+
+/* Microbenchmarks of the ceilings the kernels run against, timed with HIP
+ * events on the current device: LDS read bandwidth over all CUs, and the issue
+ * rates (wave instructions per second) of independent FP32 FMAs and of the
+ * hardware v_sin and v_exp. */
+typedef struct {
+  double lds_bytes_per_s;
+  double fma_wave_instr_per_s;
+  double sin_wave_instr_per_s;
+  double exp_wave_instr_per_s;
+  int32_t compute_units, reserved;
+} slsf_ceilings;
+int slsf_measure_ceilings(slsf_ceilings* out);
+
+/* ======================================================================== */
 /* Job configuration (TOML) and task expansion                               */
 /* ======================================================================== */
 

@@ -197,6 +197,7 @@ _SIGS = {
     "slsf_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ServerStats)]),
     "slsf_server_slot_bytes": (ctypes.c_int64, [ctypes.c_int32]),
     "slsf_device_memory": (ctypes.c_int, [_i64p, _i64p]),
+    "slsf_measure_ceilings": (ctypes.c_int, [ctypes.c_void_p]),
     "slsf_run_theta_simulation": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, ctypes.c_double, _dp, _dp,
@@ -464,6 +465,20 @@ def jackknife(loo, f_all: float):
 
 
 # ---- serving -----------------------------------------------------------------------
+class Ceilings(ctypes.Structure):
+    _fields_ = [("lds_bytes_per_s", ctypes.c_double), ("fma_wave_instr_per_s", ctypes.c_double),
+                ("sin_wave_instr_per_s", ctypes.c_double),
+                ("exp_wave_instr_per_s", ctypes.c_double), ("compute_units", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+def measure_ceilings() -> dict:
+    """Measured LDS bandwidth and FMA / v_sin / v_exp issue rates of the device."""
+    c = Ceilings()
+    _check(_lib.slsf_measure_ceilings(ctypes.byref(c)))
+    return {n: getattr(c, n) for n, _ in Ceilings._fields_ if n != "reserved"}
+
+
 def device_memory() -> tuple[int, int]:
     """(free, total) bytes of the current device."""
     f, t = ctypes.c_int64(), ctypes.c_int64()
