@@ -687,7 +687,13 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
   const auto t0 = std::chrono::steady_clock::now();
   int64_t launches = 0;
   double kernel_ms = 0.0;
-  if (b->precision == SLSF_FP32 && b->resident.covers_all) {
+  // FP32: every cubic replica of a resident size runs the resident kernel and
+  // the others the HBM kernels, so a replica's path (and stream) depends on its
+  // own size only, never on what it is batched with.  FP64 replay: HBM kernels.
+  const int64_t total = opt->thermalization + opt->sweeps;
+  const bool resident_part = b->precision == SLSF_FP32 && !b->resident.groups.empty();
+  const bool hbm_part = !resident_part || !b->resident.covers_all;
+  if (resident_part) {
     ResidentRunArgs a;
     a.thermalization = opt->thermalization;
     a.sweeps = opt->sweeps;
@@ -701,28 +707,22 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     a.bin_cap = b->bin_cap;
     a.acc_out = b->g.acc;
     a.csize_out = b->g.csize;
-    ResidentRunResult res;
+    ResidentRunResult res;  // zeroes the tallies of every replica first
     if (int rc = resident_run(b->resident, b->g, a, b->stream, &res)) return rc;
-    launches = res.launches;
-    kernel_ms = res.kernel_ms;
-    std::vector<unsigned long long> acc, cs;
-    if (int rc = read_tallies(b, b->g.acc, acc)) return rc;
-    if (int rc = read_tallies(b, (unsigned long long*)b->g.csize, cs)) return rc;
-    const int64_t nsw = opt->thermalization + opt->sweeps;
-    for (size_t r = 0; r < b->reps.size(); ++r) {
-      b->acc_sum[r] += (double)acc[r] / (double)b->reps[r].volume;
-      b->acc_count[r] += nsw;
-      b->cluster_sum[r] += (double)cs[r];
-      b->cluster_count[r] += nsw * b->reps[r].wolff_steps;
-    }
-  } else {
+    launches += res.launches;
+    kernel_ms += res.kernel_ms;
+  }
+  if (hbm_part) {
     // HBM path: tallies stay on the device until the end of the run; the
-    // replicas' windows cut thermalization (no measurement) and the tail
+    // replicas' windows cut thermalization (no measurement) and the tail, and
+    // replicas the resident kernel already ran to their window end sit out
     const int wmax = max_wolff_steps(b);
-    const int64_t total = opt->thermalization + opt->sweeps;
-    launch_clear_counters(b->g, b->stream);
+    if (!resident_part) {
+      launch_clear_counters(b->g, b->stream);
+      launches += 2;
+    }
     launch_sweep_end(b->g, 0, 0, b->stream);
-    launches += 3;
+    launches += 1;
     BinTarget bt;
     if (opt->sweeps > 0) {
       bt.accum = b->d_accum;
@@ -756,9 +756,9 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
       launches += enqueue(total);
     }
     HIP_TRY(hipGetLastError());
-    if (int rc = gather_acceptance(b, total, nullptr)) return rc;
-    if (int rc = gather_clusters(b, clusters_in(b, total), nullptr)) return rc;
   }
+  if (int rc = gather_acceptance(b, total, nullptr)) return rc;
+  if (int rc = gather_clusters(b, clusters_in(b, total), nullptr)) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   const auto t1 = std::chrono::steady_clock::now();
   if (stats) {
@@ -786,7 +786,7 @@ int slsf_autotune(slsf_batch* b, int32_t trial_sweeps, slsf_tune_record* records
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (count) *count = 0;
-  if (b->precision != SLSF_FP32 || !b->resident.covers_all)
+  if (b->precision != SLSF_FP32 || b->resident.groups.empty())
     return SLSF_OK;  // nothing to tune: the HBM kernels have no candidates
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
   std::vector<TuneRecord> recs;

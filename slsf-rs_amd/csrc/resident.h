@@ -19,7 +19,8 @@ struct ResidentGroup {
 };
 
 struct ResidentPlan {
-  bool covers_all = false;      // every replica is cubic with a resident L
+  bool covers_all = false;      // every replica is cubic with a resident L (groups
+                                // hold the ones that are; the rest take the HBM path)
   std::vector<ResidentGroup> groups;
   std::vector<hipStream_t> streams;  // one per group after the first
   std::vector<hipEvent_t> events;    // join events of those streams

@@ -550,7 +550,6 @@ int resident_plan(const std::vector<RepDev>& reps, ResidentPlan& plan) {
     if (r.lx == r.ly && r.ly == r.lz && resident_supports(r.lx)) by_l[r.lx].push_back((int32_t)i);
     else plan.covers_all = false;
   }
-  if (!plan.covers_all) return SLSF_OK;
   for (auto& kv : by_l) {
     ResidentGroup g;
     g.L = kv.first;

@@ -281,3 +281,24 @@ def test_hbm_graph_replay_matches_stream_launches(precision):
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2][0], out[1][2][0]) and np.array_equal(out[0][2][1], out[1][2][1])
     assert list(out[0][2][0]) == [28] * len(reps)
+
+
+def test_mixed_resident_and_hbm_sizes_follow_their_own_chains():
+    """FP32 batches that mix sizes the resident kernel takes with sizes it does
+    not: each replica runs the path of its own size, so its angles and bins are
+    bit-identical to the same replica run alone."""
+    reps = (reps_for(8, 2, T=1.2, seed0=300) + reps_for(10, 2, T=1.4, seed0=310) +
+            reps_for(16, 1, T=0.9, seed0=320) + reps_for(20, 1, T=1.6, seed0=330))
+    reps.append(slsf.Replica(8, 8, 12, 1.1, np.linspace(0.6, 1.8, 12), np.ones(12), 340))
+    mix = slsf.Batch(reps, slsf.FP32)
+    mix.init_angles()
+    st = mix.run(6, 12, 4)
+    assert st["site_updates"] == sum(r.volume for r in reps) * 18
+    m, w = mix.counters()
+    assert list(m) == [18] * len(reps) and list(w) == [18] * len(reps)
+    for i, r in enumerate(reps):
+        one = slsf.Batch([r], slsf.FP32)
+        one.init_angles()
+        one.run(6, 12, 4)
+        assert np.array_equal(mix.get_theta(i), one.get_theta(0)), i
+        assert np.array_equal(mix.all_bins(i), one.all_bins(0)), i
