@@ -97,3 +97,30 @@ def test_memory_budget_admission():
     with pytest.raises(slsf.SlsfError) as e:
         srv.submit(**job_args(15, 4))
     assert e.value.message == "red-black Metropolis sweep requires even lattice dimensions"
+
+
+def test_served_job_equals_batch_run():
+    """A served job (resident pool for L=16, HBM pool for L=48, chunks of 30
+    sweeps) gives exactly the bins of the same replica run through Batch.run:
+    same keyed stream, same windows, couplings from SplitMix64 U[0.6, 1.8]."""
+    import oracle_lib as orc
+    srv = slsf.Server(8 * GB, 4, chunk_sweeps=30)
+    specs = {L: job_args(L, 4242 + L, sweeps=120, therm=40, T=1.25) for L in (16, 48)}
+    for a in specs.values():
+        srv.submit(**a)
+    res = {r["l"]: r for r in srv.run_until_idle()}
+    for L, a in specs.items():
+        jxy = orc.uniform_layers(L, 0.6, 1.8, a["disorder_seed"])
+        b = slsf.Batch([slsf.Replica(L, L, L, a["temperature"], jxy, np.ones(L), a["seed"])],
+                       slsf.FP32)
+        b.init_angles()
+        b.run(a["thermalization"], 0, 1)
+        b.run(0, a["sweeps"], a["binsize"])
+        bins = b.all_bins(0)
+        r = res[L]
+        assert r["nbins"] == bins.shape[1] == a["sweeps"] // a["binsize"]
+        for k, name in enumerate(slsf.SERIES):
+            m = 0.0
+            for v in bins[k]:
+                m += v
+            assert r["series_mean"][name] == m / bins.shape[1], (L, name)
