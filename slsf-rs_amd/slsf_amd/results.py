@@ -14,6 +14,14 @@
   name with the reference's messages.
 * ``dataframe_from_result`` / ``write_dataframe`` restate src/result_tools.rs:47-
   112,160-251 (columns, ``{:.16}`` floats, quoting of delimited fields).
+* ``write_job_measurements`` / ``write_task_measurements_to_path`` follow
+  src/results.rs:38-129: one ``task{index+1:04}/run0001.meas.h5`` per task with
+  ``observables/<series>/{bin_length, samples}`` (scalar int64; float64 bin
+  means, chunks of 1000, unlimited) and ``version/{carlo_version, mc_package,
+  mc_version}`` (fixed-length strings), written through ``.h5.tmp`` and a
+  rename, with the reference's messages.  The device bins of a task travel in
+  its ``measurement_bins``, which, like the reference's ``#[serde(skip)]``
+  field, the JSON result leaves out.
 
 An estimate is written as ``{"mean", "error", "internal_bin_len"}``, the
 fields of carlo-mc's ``Estimate`` the reference itself reads
@@ -31,6 +39,7 @@ from typing import Sequence
 import numpy as np
 
 from . import (EVALS, FP32, SERIES, Batch, Job, Replica, SlsfError, evaluate)
+from . import hdf5
 
 # fields of ThetaTask (src/model.rs:20-47), in declaration order
 TASK_FIELDS = ["name", "l", "l_x", "l_y", "l_z", "temperature", "j_xy", "delta_j_xy",
@@ -100,15 +109,42 @@ def run_job(job: Job, rank: int = 0, world_size: int = 1, precision: int = FP32,
             if i != 0:
                 entry["task_index"] = int(i)  # serialised only when nonzero
             entry.update({"observables": dict(sorted(obs.items())),
-                          "acceptance": float(acceptance[k]), "measurements": sweeps})
+                          "acceptance": float(acceptance[k]), "measurements": sweeps,
+                          "measurement_bins": {name: np.array(bins[s], dtype=np.float64)
+                                               for s, name in enumerate(SERIES)
+                                               if bins.shape[1] > 0}})
             result["tasks"].append(entry)
     finally:
         b.close()
     return result
 
 
-def result_path(job_name: str, rank: int, output_dir: str) -> str:
-    return os.path.join(output_dir, f"{job_name}.rank{rank}.results.json")
+def _file_stem(job_name: str) -> str:
+    return os.path.basename(job_name) or job_name
+
+
+def _output_dir(job_name: str, output_dir: str | None) -> str:
+    """src/model.rs:948-995: the given directory, else $XY_OUTPUT_DIR, else the
+    job name's parent directory, else ``results``."""
+    if output_dir is not None:
+        return output_dir
+    env = os.environ.get("XY_OUTPUT_DIR")
+    if env is not None:
+        return env
+    return os.path.dirname(job_name) or "results"
+
+
+def result_path(job_name: str, rank: int, output_dir: str | None = None) -> str:
+    return os.path.join(_output_dir(job_name, output_dir),
+                        f"{_file_stem(job_name)}.rank{rank}.results.json")
+
+
+def merged_result_path(job_name: str, output_dir: str | None = None) -> str:
+    return os.path.join(_output_dir(job_name, output_dir), f"{_file_stem(job_name)}.results.json")
+
+
+def measurement_dir(job_name: str, output_dir: str | None = None) -> str:
+    return os.path.join(_output_dir(job_name, output_dir), f"{_file_stem(job_name)}.data")
 
 
 def _finite(x):
@@ -124,16 +160,63 @@ def _finite(x):
 
 def write_job_result(result: dict, output_dir: str) -> str:
     os.makedirs(output_dir, exist_ok=True)
-    return write_job_result_to_path(result, result_path(result["job_name"], result["rank"],
-                                                        output_dir))
+    return write_job_result_to_path(result, os.path.join(
+        output_dir, f"{result['job_name']}.rank{result['rank']}.results.json"))
 
 
 def write_job_result_to_path(result: dict, path: str) -> str:
     parent = os.path.dirname(path)
     if parent:
         os.makedirs(parent, exist_ok=True)
+    tasks = [{k: v for k, v in t.items() if k != "measurement_bins"} for t in result["tasks"]]
     with open(path, "w") as f:
-        json.dump(_finite(result), f, indent=2)
+        json.dump(_finite(dict(result, tasks=tasks)), f, indent=2)
+    return path
+
+
+def write_job_measurements(result: dict, output_dir: str) -> list:
+    """src/results.rs:38-54: one measurement file per task."""
+    paths = []
+    for tr in result["tasks"]:
+        task_dir = os.path.join(output_dir, f"task{tr.get('task_index', 0) + 1:04d}")
+        os.makedirs(task_dir, exist_ok=True)
+        path = os.path.join(task_dir, "run0001.meas.h5")
+        write_task_measurements_to_path(tr, path)
+        paths.append(path)
+    return paths
+
+
+def write_task_measurements_to_path(task_result: dict, path: str) -> str:
+    """src/results.rs:56-129, the same groups, datasets and error messages."""
+    bins = task_result.get("measurement_bins") or {}
+    if not bins:
+        raise ValueError("theta task result does not contain measurement bins")
+    observables = {}
+    for name in sorted(bins):
+        samples = np.asarray(bins[name], dtype=np.float64)
+        est = task_result["observables"].get(name)
+        if est is None:
+            raise ValueError(f"missing observable estimate for {name}")
+        if int(est["internal_bin_len"]) == 0:
+            raise ValueError(f"{name}: internal bin length must be positive")
+        if samples.size == 0:
+            raise ValueError(f"{name}: no complete measurement bins to write")
+        observables[name] = {
+            "bin_length": hdf5.Dataset("i8", int(est["internal_bin_len"]), shape=()),
+            "samples": hdf5.Dataset("f8", samples, maxshape=(hdf5.UNDEF,), chunks=(1000,)),
+        }
+    version = {
+        "carlo_version": hdf5.Dataset("S", os.environ.get("XY_CARLO_VERSION", "0.3.4")),
+        "mc_package": hdf5.Dataset("S", "SLSF.XYCarlo"),
+        "mc_version": hdf5.Dataset("S", os.environ.get("XY_MC_VERSION", "0.1.0")),
+    }
+    parent = os.path.dirname(path)
+    if parent:
+        os.makedirs(parent, exist_ok=True)
+    stem, ext = os.path.splitext(path)
+    tmp = (stem if ext else path) + ".h5.tmp"
+    hdf5.write_file(tmp, {"observables": observables, "version": version})
+    os.replace(tmp, path)
     return path
 
 

@@ -55,3 +55,25 @@ def test_ranks_split_and_merge():
         assert t["observables"]["Energy"] == by_name[t["task"]["name"]]["observables"]["Energy"]
     with pytest.raises(slsf.SlsfError):
         R.run_job(job, rank=3, world_size=3)
+
+
+def test_run_job_measurement_files(tmp_path):
+    """A GPU run's device bins go to the reference's measurement files
+    (task{n}/run0001.meas.h5); read back bit-exactly when h5dump is present."""
+    import os
+    import shutil
+    import subprocess
+    job = slsf.Job(JOB)
+    res = R.run_job(job, rank=1, world_size=4)
+    paths = R.write_job_measurements(res, R.measurement_dir(job.name, str(tmp_path)))
+    assert len(paths) == 2
+    for tr, p in zip(res["tasks"], paths):
+        assert p.endswith(os.path.join(f"task{tr['task_index'] + 1:04d}", "run0001.meas.h5"))
+        assert sorted(tr["measurement_bins"]) == sorted(slsf.SERIES)
+        assert all(len(v) == 6 for v in tr["measurement_bins"].values())
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if os.path.exists(h5dump):
+        out = tmp_path / "raw.bin"
+        subprocess.run([h5dump, "-d", "/observables/Energy/samples", "-b", "LE", "-o", str(out),
+                        paths[1]], check=True, capture_output=True)
+        assert out.read_bytes() == res["tasks"][1]["measurement_bins"]["Energy"].tobytes()
