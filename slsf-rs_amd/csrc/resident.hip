@@ -21,7 +21,7 @@
 // (y, z) row (wave ballots).  The cluster is the seed's component of that bond
 // graph -- the same set the reference's DFS builds from the same uniforms --
 // found by a monotone flood fill over rows (x-runs closed inside the word by
-// a doubling closure, y and z through the words 1, 2 and 4 rows away along
+// carry arithmetic, y and z through the words 1, 2 and 4 rows away along
 // straight bonded lines) until a round changes nothing.  Members are reflected
 // in registers.
 //
@@ -76,28 +76,57 @@ struct ResArgs {
 };
 
 
+// Rotations and reversal of an L-bit periodic row held in the low bits of a
+// word (L = 32: v_alignbit_b32 and v_bfrev_b32).
 template <int L>
-SLSF_D uint32_t rotl_s(uint32_t w, int s) {
+SLSF_D uint32_t rotr_row(uint32_t w, uint32_t s) {
+  if (L == 32) return __builtin_amdgcn_alignbit(w, w, s);
+  return ((w >> s) | (w << (L - s))) & RShape<L, 1>::MASK;
+}
+template <int L>
+SLSF_D uint32_t rotl_row(uint32_t w, uint32_t s) {
+  if (L == 32) return __builtin_amdgcn_alignbit(w, w, 32u - s);
   return ((w << s) | (w >> (L - s))) & RShape<L, 1>::MASK;
 }
 template <int L>
-SLSF_D uint32_t rotr_s(uint32_t w, int s) {
-  return ((w >> s) | (w << (L - s))) & RShape<L, 1>::MASK;
+SLSF_D uint32_t rev_row(uint32_t w) {
+  return __builtin_bitreverse32(w) >> (32 - L);
 }
-// Closure of the marked sites n of a periodic row through its active x-bonds
-// b (bit i: sites i and i+1 mod L bonded) by doubling: P_s(i) = sites i and
-// i+s joined by a chain of bonds, P_2s = P_s & rotr(P_s, s); with s = 1, 2,
-// 4, ... < L every distance d < L is a sum of increasing steps, so every site
-// of a bonded run that holds a mark ends up marked.
+
+// Runs of a periodic row under its active x-bonds b (bit i: sites i and i+1
+// mod L bonded), prepared once per cluster for run_closure.  The row is
+// rotated right by s so that an inactive bond sits at position L-1 (with
+// every bond active, bond L-1 is cut: the chain 0..L-1 is still one run);
+// br are the rotated bonds, eu their run tops (~br), and bd/ed the same for
+// the bit-reversed row, which turns downward propagation into upward.
+struct RowRuns {
+  uint32_t s, br, eu, bd, ed;
+};
 template <int L>
-SLSF_D uint32_t row_closure(uint32_t n, uint32_t b) {
-  uint32_t p = b;
-#pragma unroll
-  for (int s = 1; s < L; s *= 2) {
-    n |= rotl_s<L>(n & p, s) | (rotr_s<L>(n, s) & p);
-    p &= rotr_s<L>(p, s);
-  }
-  return n;
+SLSF_D RowRuns row_runs(uint32_t b) {
+  constexpr uint32_t M = RShape<L, 1>::MASK;
+  const uint32_t open = ~b & M;
+  const uint32_t j = open ? (uint32_t)__builtin_ctz(open) : (uint32_t)(L - 1);
+  RowRuns r;
+  r.s = (j + 1 == (uint32_t)L) ? 0u : j + 1;
+  r.br = rotr_row<L>(b, r.s) & (M >> 1);
+  r.eu = ~r.br & M;
+  r.bd = rev_row<L>(r.br) >> 1;
+  r.ed = ~r.bd & M;
+  return r;
+}
+// Closure of the marked sites n through the row's runs.  Upward, with the
+// seeds q & br (seeds at run tops move nowhere up): eu - seeds borrows from
+// each seed up to the top of its run and stops there (every run, the last
+// included, ends at a bit of eu), so (eu - seeds) ^ eu is every site from a
+// run's lowest seed to its top; the reversed row does the same downward.
+template <int L>
+SLSF_D uint32_t run_closure(uint32_t n, const RowRuns& r) {
+  uint32_t q = rotr_row<L>(n, r.s);
+  q |= (r.eu - (q & r.br)) ^ r.eu;
+  uint32_t v = rev_row<L>(q);
+  v |= (r.ed - (v & r.bd)) ^ r.ed;
+  return rotl_row<L>(rev_row<L>(v), r.s);
 }
 
 // An empty asm makes a value opaque to the optimizer.  sel() makes both
@@ -434,11 +463,22 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       pz4[r] = pz2[r] & pz2[ry + L * ((rz + 2) % L)];
     }
     __syncthreads();
+    // the runs of this thread's rows stay in registers for the whole fill
+    constexpr int RPT = (S::ROWS + S::NT - 1) / S::NT;
+    RowRuns runs[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = t + i * S::NT;
+      runs[i] = row_runs<L>(r < S::ROWS ? bxm[r] : 0u);
+    }
     // flood fill to the fixed point; the flag of round r lives in slot r % 3,
     // which thread 0 clears only in round r + 2, after every thread read it
     for (int round = 0;; ++round) {
       bool changed = false;
-      for (int r = t; r < S::ROWS; r += S::NT) {
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int r = t + i * S::NT;
+        if (RPT * S::NT > S::ROWS && r >= S::ROWS) break;
         const int ry = r % L, rz = r / L;
         const int rowyp = ((ry + 1) % L) + L * rz, rowym = ((ry + L - 1) % L) + L * rz;
         const int rowzp = ry + L * ((rz + 1) % L), rowzm = ry + L * ((rz + L - 1) % L);
@@ -453,7 +493,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         const int z4m = ry + L * ((rz + L - 4) % L), z4p = ry + L * ((rz + 4) % L);
         n |= (vis[y4m] & py4[y4m]) | (vis[y4p] & py4[r]) | (vis[z4m] & pz4[z4m]) |
              (vis[z4p] & pz4[r]);
-        n = row_closure<L>(n, bxm[r]);
+        n = run_closure<L>(n, runs[i]);
         if (n != v) {
           vis[r] = n;
           changed = true;
