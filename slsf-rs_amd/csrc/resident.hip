@@ -129,19 +129,15 @@ SLSF_D uint32_t run_closure(uint32_t n, const RowRuns& r) {
   return rotl_row<L>(rev_row<L>(v), r.s);
 }
 
-// An empty asm makes a value opaque to the optimizer.  sel() makes both
-// operands opaque before selecting, so `cond ? a[i] : a[j]` stays a select of
-// two registers instead of a load from a selected address (which would move
-// the register array to scratch), and coordinates made opaque at the start of
-// every phase keep per-site address arithmetic from being hoisted out of the
-// sweep loop into hundreds of live registers.
+
+// An empty asm makes a value opaque to the optimizer.  Every phase reads its
+// angles through opq(): otherwise the compiler rewrites cos(acc ? t1 : t0) as
+// acc ? cos(t1) : cos(t0) and keeps the Metropolis step's sines and cosines
+// alive from one phase to the next, which the register budget of the
+// 1024-thread L=32 workgroup cannot hold.
 SLSF_D float opq(float v) {
   asm("" : "+v"(v));
   return v;
-}
-SLSF_D float sel(bool cond, float a, float b) {
-  const float oa = opq(a), ob = opq(b);
-  return cond ? oa : ob;
 }
 
 // hardware transcendentals: v_sin/v_cos take revolutions, v_exp is 2^x
@@ -196,15 +192,27 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   const uint32_t k0 = d.key0, k1 = d.key1;
   uint32_t mctr = d.metro_ctr, wctr = d.wolff_ctr;
 
-  // angles of this thread's column in turns, [0, 1) (the FP32 state format)
+  // Angles of this thread's columns in turns, [0, 1) (the FP32 state
+  // format), by colour: th[j][c * L/2 + k] is the colour-c site of column j,
+  // at z = 2k + (c ^ zpar) with zpar = (x + y) & 1 (one parity for all of a
+  // thread's columns, YS being even), so every colour phase names its sites
+  // at compile time.  Coordinates are made opaque (empty asm) at the start of
+  // every phase, which keeps per-site address arithmetic from being hoisted
+  // out of the sweep loop into hundreds of live registers.
+  constexpr int H = L / 2;
   float th[CPT][L];
+  {
+    const int zpar = (x + y) & 1;
 #pragma unroll
-  for (int j = 0; j < CPT; ++j)
+    for (int j = 0; j < CPT; ++j)
 #pragma unroll
-    for (int z = 0; z < L; ++z) {
-      th[j][z] = 0.0f;
-      if (own) th[j][z] = a.theta[base + x + L * (y + j * S::YS + L * z)];
-    }
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          const int z = 2 * k + (c ^ zpar);
+          th[j][c * H + k] = own ? a.theta[base + x + L * (y + j * S::YS + L * z)] : 0.0f;
+        }
+  }
   if (t < L) {
     const int zm = (t == 0) ? L - 1 : t - 1;
     jt[t] = make_float4(a.jxy[d.jz_off + t], a.jz[d.jz_off + t], a.jz[d.jz_off + zm], 0.0f);
@@ -228,9 +236,10 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     for (int j = 0; j < CPT; ++j) {
       const int b = ((x + L * (y + j * S::YS)) >> 1) + HL * z1;
 #pragma unroll
-      for (int k = 0; k < L / 2; ++k) {
-        const float tv = sel(z1, th[j][2 * k + 1], th[j][2 * k]);
+      for (int k = 0; k < H; ++k) {
+        const float tv = opq(th[j][col * H + k]);
         xbuf[b + 2 * HL * k] = make_float2(cos_t(tv), sin_t(tv));
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
   };
@@ -255,8 +264,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const int bz = ((x + L * y) >> 1) + s0;
 #pragma unroll
     for (int k = 0; k < L / 2; ++k) {
-      const int ka = 2 * k, kb = 2 * k + 1;
-      const float t0 = sel(z0, th[j][kb], th[j][ka]);
+      const float t0 = opq(th[j][p * H + k]);
       const int zo = 2 * HL * k;  // plane offset of z = 2k + z0
       // z neighbours, periodic: z = L - 1 (last k, z0 = 1) wraps up to 0,
       // z = 0 (first k, z0 = 0) wraps down to L - 1
@@ -288,9 +296,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
       const bool acc = (de <= 0.0f) | (ua < pacc);
       nacc_wave += __popcll(__ballot(acc));  // scalar count per wave
-      const bool wa = acc & !z0, wb = acc & (z0 != 0);
-      th[j][ka] = wa ? t1 : th[j][ka];
-      th[j][kb] = wb ? t1 : th[j][kb];
+      th[j][p * H + k] = acc ? t1 : t0;
       // one site in flight per thread: keeps the register budget of the
       // 1024-thread L=32 workgroup (occupancy supplies the latency hiding)
       __builtin_amdgcn_sched_barrier(0);
@@ -318,8 +324,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       const int bz = ((x + L * y) >> 1) + s0;
 #pragma unroll
       for (int k = 0; k < L / 2; ++k) {
-        const int ka = 2 * k, kb = 2 * k + 1;
-        const float t0 = sel(z0, th[j][kb], th[j][ka]);
+        const float t0 = opq(th[j][k]);
         const float cc = cos_t(t0), ss = sin_t(t0);
         const int zo = 2 * HL * k;
         const float2 nxp = xbuf[bxp + zo];
@@ -377,11 +382,17 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const int seed = (int)(((uint64_t)ws.w2 * (uint64_t)S::V) >> 32);
     // r_i = cos(theta_i - phi) (src/updates.rs:579-582)
     if (own) {
+      const int zpar = (x + y) & 1;
 #pragma unroll
       for (int j = 0; j < CPT; ++j)
 #pragma unroll
-        for (int z = 0; z < L; ++z)
-          rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(th[j][z] - phit);
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            const int z = 2 * k + (c ^ zpar);
+            rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(opq(th[j][c * H + k]) - phit);
+            __builtin_amdgcn_sched_barrier(0);
+          }
     }
     __syncthreads();
     const int wave = t >> 6, lane = t & 63;
@@ -507,13 +518,19 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     // reflection theta -> 2 phi + pi - theta (src/updates.rs:503-506), in turns
     if (own) {
       const float refl = 2.0f * phit + 0.5f;
+      const int zpar = (x + y) & 1;
 #pragma unroll
       for (int j = 0; j < CPT; ++j)
 #pragma unroll
-        for (int z = 0; z < L; ++z) {
-          const bool in = (vis[y + j * S::YS + L * z] >> x) & 1u;
-          th[j][z] = in ? fract_hw(refl - th[j][z]) : th[j][z];
-        }
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int k = 0; k < H; ++k) {
+            const int z = 2 * k + (c ^ zpar);
+            const bool in = (vis[y + j * S::YS + L * z] >> x) & 1u;
+            float& v = th[j][c * H + k];
+            v = in ? fract_hw(refl - v) : v;
+            __builtin_amdgcn_sched_barrier(0);
+          }
     }
     int members = 0;
     for (int r = t; r < S::ROWS; r += S::NT) members += __popc(vis[r]);
@@ -550,14 +567,19 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
 
   // write back the angles (turns) and their trig cache, counters, tallies
   if (own) {
+    const int zpar = (x + y) & 1;
 #pragma unroll
     for (int j = 0; j < CPT; ++j)
 #pragma unroll
-      for (int z = 0; z < L; ++z) {
-        const int64_t g = base + x + L * (y + j * S::YS + L * z);
-        a.theta[g] = th[j][z];
-        a.cs[g] = make_float2(cos_t(th[j][z]), sin_t(th[j][z]));
-      }
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+          const int z = 2 * k + (c ^ zpar);
+          const int64_t g = base + x + L * (y + j * S::YS + L * z);
+          const float v = opq(th[j][c * H + k]);
+          a.theta[g] = v;
+          a.cs[g] = make_float2(cos_t(v), sin_t(v));
+        }
   }
   double tallies[2] = {(t & 63) == 0 ? (double)nacc_wave : 0.0, (double)ncluster};
   __syncthreads();
