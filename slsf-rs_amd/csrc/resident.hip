@@ -76,17 +76,12 @@ struct ResArgs {
 };
 
 
-// Rotations and reversal of an L-bit periodic row held in the low bits of a
-// word (L = 32: v_alignbit_b32 and v_bfrev_b32).
+// Rotation (right by s < L) and reversal of an L-bit periodic row held in the
+// low bits of a word (L = 32: v_alignbit_b32 and v_bfrev_b32).
 template <int L>
 SLSF_D uint32_t rotr_row(uint32_t w, uint32_t s) {
   if (L == 32) return __builtin_amdgcn_alignbit(w, w, s);
   return ((w >> s) | (w << (L - s))) & RShape<L, 1>::MASK;
-}
-template <int L>
-SLSF_D uint32_t rotl_row(uint32_t w, uint32_t s) {
-  if (L == 32) return __builtin_amdgcn_alignbit(w, w, 32u - s);
-  return ((w << s) | (w >> (L - s))) & RShape<L, 1>::MASK;
 }
 template <int L>
 SLSF_D uint32_t rev_row(uint32_t w) {
@@ -100,7 +95,7 @@ SLSF_D uint32_t rev_row(uint32_t w) {
 // br are the rotated bonds, eu their run tops (~br), and bd/ed the same for
 // the bit-reversed row, which turns downward propagation into upward.
 struct RowRuns {
-  uint32_t s, br, eu, bd, ed;
+  uint32_t s, back, br, eu, bd, ed;  // back = (L - s) % L undoes the rotation
 };
 template <int L>
 SLSF_D RowRuns row_runs(uint32_t b) {
@@ -109,6 +104,7 @@ SLSF_D RowRuns row_runs(uint32_t b) {
   const uint32_t j = open ? (uint32_t)__builtin_ctz(open) : (uint32_t)(L - 1);
   RowRuns r;
   r.s = (j + 1 == (uint32_t)L) ? 0u : j + 1;
+  r.back = r.s ? (uint32_t)L - r.s : 0u;
   r.br = rotr_row<L>(b, r.s) & (M >> 1);
   r.eu = ~r.br & M;
   r.bd = rev_row<L>(r.br) >> 1;
@@ -126,7 +122,7 @@ SLSF_D uint32_t run_closure(uint32_t n, const RowRuns& r) {
   q |= (r.eu - (q & r.br)) ^ r.eu;
   uint32_t v = rev_row<L>(q);
   v |= (r.ed - (v & r.bd)) ^ r.ed;
-  return rotl_row<L>(rev_row<L>(v), r.s);
+  return rotr_row<L>(rev_row<L>(v), r.back);
 }
 
 
@@ -474,13 +470,38 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       pz4[r] = pz2[r] & pz2[ry + L * ((rz + 2) % L)];
     }
     __syncthreads();
-    // the runs of this thread's rows stay in registers for the whole fill
+    // Everything about a row that the rounds leave unchanged stays in
+    // registers for the whole fill: its own marks (only this thread writes
+    // them), its runs, and for the 12 rows a mark can come from (1, 2 and 4
+    // rows away in y and in z, both ways) that row's index and the bond or
+    // path mask that lets the mark through.
     constexpr int RPT = (S::ROWS + S::NT - 1) / S::NT;
+    constexpr int NB = 12;
     RowRuns runs[RPT];
+    const uint32_t* nb_row[RPT][NB];
+    uint32_t nb_mask[RPT][NB], mine[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = t + i * S::NT;
-      runs[i] = row_runs<L>(r < S::ROWS ? bxm[r] : 0u);
+      const bool valid = r < S::ROWS;
+      const int rr = valid ? r : 0;
+      const int ry = rr % L, rz = rr / L;
+      const int ym = ((ry + L - 1) % L) + L * rz, yp = ((ry + 1) % L) + L * rz;
+      const int zm = ry + L * ((rz + L - 1) % L), zp = ry + L * ((rz + 1) % L);
+      const int y2m = ((ry + L - 2) % L) + L * rz, y2p = ((ry + 2) % L) + L * rz;
+      const int z2m = ry + L * ((rz + L - 2) % L), z2p = ry + L * ((rz + 2) % L);
+      const int y4m = ((ry + L - 4) % L) + L * rz, y4p = ((ry + 4) % L) + L * rz;
+      const int z4m = ry + L * ((rz + L - 4) % L), z4p = ry + L * ((rz + 4) % L);
+      const int rows[NB] = {ym, yp, zm, zp, y2m, y2p, z2m, z2p, y4m, y4p, z4m, z4p};
+      const uint32_t masks[NB] = {bym[ym], bym[rr], bzm[zm], bzm[rr], py2[y2m], py2[rr],
+                                  pz2[z2m], pz2[rr], py4[y4m], py4[rr], pz4[z4m], pz4[rr]};
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        nb_row[i][b] = vis + rows[b];
+        nb_mask[i][b] = valid ? masks[b] : 0u;
+      }
+      runs[i] = row_runs<L>(valid ? bxm[rr] : 0u);
+      mine[i] = valid ? vis[rr] : 0u;
     }
     // flood fill to the fixed point; the flag of round r lives in slot r % 3,
     // which thread 0 clears only in round r + 2, after every thread read it
@@ -490,23 +511,13 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       for (int i = 0; i < RPT; ++i) {
         const int r = t + i * S::NT;
         if (RPT * S::NT > S::ROWS && r >= S::ROWS) break;
-        const int ry = r % L, rz = r / L;
-        const int rowyp = ((ry + 1) % L) + L * rz, rowym = ((ry + L - 1) % L) + L * rz;
-        const int rowzp = ry + L * ((rz + 1) % L), rowzm = ry + L * ((rz + L - 1) % L);
-        const uint32_t v = vis[r];
-        uint32_t n = v | (vis[rowym] & bym[rowym]) | (vis[rowyp] & bym[r]) |
-                     (vis[rowzm] & bzm[rowzm]) | (vis[rowzp] & bzm[r]);
-        const int y2m = ((ry + L - 2) % L) + L * rz, y2p = ((ry + 2) % L) + L * rz;
-        const int z2m = ry + L * ((rz + L - 2) % L), z2p = ry + L * ((rz + 2) % L);
-        n |= (vis[y2m] & py2[y2m]) | (vis[y2p] & py2[r]) | (vis[z2m] & pz2[z2m]) |
-             (vis[z2p] & pz2[r]);
-        const int y4m = ((ry + L - 4) % L) + L * rz, y4p = ((ry + 4) % L) + L * rz;
-        const int z4m = ry + L * ((rz + L - 4) % L), z4p = ry + L * ((rz + 4) % L);
-        n |= (vis[y4m] & py4[y4m]) | (vis[y4p] & py4[r]) | (vis[z4m] & pz4[z4m]) |
-             (vis[z4p] & pz4[r]);
+        uint32_t n = mine[i];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) n |= *nb_row[i][b] & nb_mask[i][b];
         n = run_closure<L>(n, runs[i]);
-        if (n != v) {
+        if (n != mine[i]) {
           vis[r] = n;
+          mine[i] = n;
           changed = true;
         }
       }
