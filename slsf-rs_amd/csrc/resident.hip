@@ -25,10 +25,12 @@
 // straight bonded lines) until a round changes nothing.  Members are reflected
 // in registers.
 //
-// Measurement (src/observables.rs:80-154): after the colour-1 exchange of the
-// next sweep, each colour-0 site sums its six bonds (each bond has exactly one
-// colour-0 end), all sites sum (cos, sin); FP32 per thread, FP64 across the
-// workgroup, accumulated into FP64 bins on the device.
+// Measurement (src/observables.rs:80-154): taken in the colour-0 Metropolis
+// pass of the next sweep, which already reads every colour-0 site's six
+// neighbours and computes its (cos, sin) before moving it: each colour-0 site
+// sums its six bonds (each bond has exactly one colour-0 end), all sites sum
+// (cos, sin); FP32 per thread, FP64 across the workgroup, accumulated into
+// FP64 bins on the device.
 //
 // Uniforms are the library's keyed Philox stream (same counters as the FP64
 // replay path; FP32 uses the top 24 bits of the words), so a replica's chain
@@ -163,6 +165,28 @@ SLSF_D void wg_sum(double (&v)[N], double* red) {
     }
 }
 
+// FP32 partial sums of one measurement over a thread's colour-0 sites: every
+// bond has exactly one colour-0 end, so each colour-0 site (cos, sin) = (c, s)
+// adds its six bonds: cos(a - b) = ca cb + sa sb summed per direction, and
+// sin along +e, sin(t_i - t_{i+e}) + sin(t_{i-e} - t_i) = s (c+ - c-) -
+// c (s+ - s-); the magnetization takes the site and its colour-1 z+ neighbour.
+struct MeasPart {
+  float ec[3], es[3], mx, my;
+};
+SLSF_D void measure_site(MeasPart& m, float c, float s, float2 nxp, float2 nxm, float2 nyp,
+                         float2 nym, float2 nzp, float2 nzm, float4 jj) {
+  const float hzc = jj.y * nzp.x + jj.z * nzm.x, hzs = jj.y * nzp.y + jj.z * nzm.y;
+  const float dzc = jj.y * nzp.x - jj.z * nzm.x, dzs = jj.y * nzp.y - jj.z * nzm.y;
+  m.ec[0] += jj.x * (c * (nxp.x + nxm.x) + s * (nxp.y + nxm.y));
+  m.ec[1] += jj.x * (c * (nyp.x + nym.x) + s * (nyp.y + nym.y));
+  m.ec[2] += c * hzc + s * hzs;
+  m.es[0] += jj.x * (s * (nxp.x - nxm.x) - c * (nxp.y - nxm.y));
+  m.es[1] += jj.x * (s * (nyp.x - nym.x) - c * (nyp.y - nym.y));
+  m.es[2] += s * dzc - c * dzs;
+  m.mx += c + nzp.x;
+  m.my += s + nzp.y;
+}
+
 // PAIRED: one Philox call serves two same-colour sites of a column (production
 // stream, tag METRO_PAIR, counter index = first site of the pair); otherwise
 // one call per site with the replay path's counters (tag METRO), which lets an
@@ -240,8 +264,11 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     }
   };
 
-  // Metropolis update of this thread's colour-p sites
-  auto update = [&](int p) __attribute__((always_inline)) {
+  // Metropolis update of this thread's colour-p sites; with `meas` (colour 0
+  // only) the same pass also takes the measurement of the state it starts
+  // from, sharing the neighbour reads and the sites' (cos, sin)
+  MeasPart part;
+  auto update = [&](int p, bool meas) __attribute__((always_inline)) {
     asm volatile("" : "+v"(x), "+v"(y));
     const int y0 = y;
     const int z0 = (p + x + y) & 1;
@@ -288,6 +315,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       }
       const float t1 = fract_hw(t0 + wturn * (2.0f * up - 1.0f));
       const float c0 = cos_t(t0), s0 = sin_t(t0), c1 = cos_t(t1), s1 = sin_t(t1);
+      if (meas) measure_site(part, c0, s0, nxp, nxm, nyp, nym, nzp, nzm, jj);
       const float de = (c0 - c1) * hx + (s0 - s1) * hy;
       const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
       const bool acc = (de <= 0.0f) | (ua < pacc);
@@ -300,55 +328,11 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     }
   };
 
-  // observables of the current state; needs the colour-1 exchange in LDS
-  auto measure = [&]() __attribute__((always_inline)) {
-    asm volatile("" : "+v"(x), "+v"(y));
-    float e_c[3] = {0.f, 0.f, 0.f}, e_s[3] = {0.f, 0.f, 0.f};
-    float mx = 0.f, my = 0.f;
-    if (own) {
-      const int z0 = (x + y) & 1;  // colour-0 sites of the column: z = 2k + z0
-      const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
-      const float4* jz0 = jt + z0;
-      const int y0 = y;
-#pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-      const int y = y0 + j * S::YS;
-      const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
-      const int s0 = HL * z0;
-      const int bxp = ((xp + L * y) >> 1) + s0, bxm = ((xm + L * y) >> 1) + s0;
-      const int byp = ((x + L * yp) >> 1) + s0, bym = ((x + L * ym) >> 1) + s0;
-      const int bz = ((x + L * y) >> 1) + s0;
-#pragma unroll
-      for (int k = 0; k < L / 2; ++k) {
-        const float t0 = opq(th[j][k]);
-        const float cc = cos_t(t0), ss = sin_t(t0);
-        const int zo = 2 * HL * k;
-        const float2 nxp = xbuf[bxp + zo];
-        const float2 nxm = xbuf[bxm + zo];
-        const float2 nyp = xbuf[byp + zo];
-        const float2 nym = xbuf[bym + zo];
-        // colour-1 neighbours in z, read back from the exchange (periodic)
-        const int dzp = (k == L / 2 - 1) ? (z0 ? -HL * (L - 1) : HL) : HL;
-        const int dzm = (k == 0) ? (z0 ? -HL : HL * (L - 1)) : -HL;
-        const float2 nzp = xbuf[bz + zo + dzp];
-        const float2 nzm = xbuf[bz + zo + dzm];
-        const float4 jj = jz0[2 * k];
-        // bond (a -> a + e): cos = ca cb + sa sb, sin = sa cb - ca sb
-        e_c[0] += jj.x * ((cc * nxp.x + ss * nxp.y) + (nxm.x * cc + nxm.y * ss));
-        e_s[0] += jj.x * ((ss * nxp.x - cc * nxp.y) + (nxm.y * cc - nxm.x * ss));
-        e_c[1] += jj.x * ((cc * nyp.x + ss * nyp.y) + (nym.x * cc + nym.y * ss));
-        e_s[1] += jj.x * ((ss * nyp.x - cc * nyp.y) + (nym.y * cc - nym.x * ss));
-        e_c[2] += jj.y * (cc * nzp.x + ss * nzp.y) + jj.z * (nzm.x * cc + nzm.y * ss);
-        e_s[2] += jj.y * (ss * nzp.x - cc * nzp.y) + jj.z * (nzm.y * cc - nzm.x * ss);
-        mx += cc + nzp.x;  // the colour-0 site and its colour-1 z+ neighbour
-        my += ss + nzp.y;
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      }
-    }
-    double v[9] = {-(double)(e_c[0] + e_c[1] + e_c[2]), (double)mx, (double)my,
-                   (double)e_c[0], (double)e_c[1], (double)e_c[2],
-                   (double)e_s[0], (double)e_s[1], (double)e_s[2]};
+  // folds the partial sums of one measurement into the replica's FP64 bins
+  auto measure_finish = [&]() __attribute__((always_inline)) {
+    double v[9] = {-(double)(part.ec[0] + part.ec[1] + part.ec[2]), (double)part.mx,
+                   (double)part.my, (double)part.ec[0], (double)part.ec[1], (double)part.ec[2],
+                   (double)part.es[0], (double)part.es[1], (double)part.es[2]};
     wg_sum<9, S::NW>(v, red);
     if (t == 0) {
       const double vol = (double)S::V;
@@ -369,6 +353,45 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         st[1] += 1;
       }
     }
+  };
+
+  // a measurement on its own (the last sweep of a launch); needs the colour-1
+  // exchange in LDS and sums exactly what the fused pass sums
+  auto measure = [&]() __attribute__((always_inline)) {
+    asm volatile("" : "+v"(x), "+v"(y));
+    part = MeasPart{};
+    if (own) {
+      const int z0 = (x + y) & 1;  // colour-0 sites of the column: z = 2k + z0
+      const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+      const float4* jz0 = jt + z0;
+      const int y0 = y;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+      const int y = y0 + j * S::YS;
+      const int yp = (y + 1 == L) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+      const int s0 = HL * z0;
+      const int bxp = ((xp + L * y) >> 1) + s0, bxm = ((xm + L * y) >> 1) + s0;
+      const int byp = ((x + L * yp) >> 1) + s0, bym = ((x + L * ym) >> 1) + s0;
+      const int bz = ((x + L * y) >> 1) + s0;
+#pragma unroll
+      for (int k = 0; k < L / 2; ++k) {
+        const float t0 = opq(th[j][k]);
+        const int zo = 2 * HL * k;
+        const float2 nxp = xbuf[bxp + zo];
+        const float2 nxm = xbuf[bxm + zo];
+        const float2 nyp = xbuf[byp + zo];
+        const float2 nym = xbuf[bym + zo];
+        // colour-1 neighbours in z, read back from the exchange (periodic)
+        const int dzp = (k == L / 2 - 1) ? (z0 ? -HL * (L - 1) : HL) : HL;
+        const int dzm = (k == 0) ? (z0 ? -HL : HL * (L - 1)) : -HL;
+        const float2 nzp = xbuf[bz + zo + dzp];
+        const float2 nzm = xbuf[bz + zo + dzm];
+        measure_site(part, cos_t(t0), sin_t(t0), nxp, nxm, nyp, nym, nzp, nzm, jz0[2 * k]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      }
+    }
+    measure_finish();
   };
 
   auto wolff = [&]() __attribute__((always_inline)) {
@@ -556,12 +579,17 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   for (int sw = 0; sw < a.nsweeps && mctr < d.stop_at; ++sw) {
     exchange(1);
     __syncthreads();
-    if (pending) measure();
-    update(0);
+    if (pending) {
+      part = MeasPart{};
+      update(0, true);
+      measure_finish();
+    } else {
+      update(0, false);
+    }
     __syncthreads();
     exchange(0);
     __syncthreads();
-    update(1);
+    update(1, false);
     __syncthreads();
     pending = mctr >= d.measure_from;
     mctr += 1u;
