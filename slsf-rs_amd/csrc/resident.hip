@@ -138,6 +138,11 @@ SLSF_D float opq(float v) {
   return v;
 }
 
+// The 24-bit integer of a uniform word as a float (exact): u24(w) = w24(w) 2^-24.
+// Comparisons and affine maps of u are rewritten on w24 where the scaling by
+// a power of two stays exact, which saves the multiply and keeps every value.
+SLSF_D float w24(uint32_t w) { return (float)(w >> 8); }
+
 // hardware transcendentals: v_sin/v_cos take revolutions, v_exp is 2^x
 SLSF_D float sin_t(float t) { return __builtin_amdgcn_sinf(t); }
 SLSF_D float cos_t(float t) { return __builtin_amdgcn_cosf(t); }
@@ -303,17 +308,18 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       const int row = L * (y + L * (2 * k + z0));
       const float hx = jj.x * (nxp.x + nxm.x + nyp.x + nym.x) + jj.y * nzp.x + jj.z * nzm.x;
       const float hy = jj.x * (nxp.y + nxm.y + nyp.y + nym.y) + jj.y * nzp.y + jj.z * nzm.y;
-      float up, ua;
+      float wp, ua;  // wp: the proposal's 24-bit integer as a float
       if (PAIRED) {
         if (k % 2 == 0) w = keyed(k0, k1, (uint32_t)(x + row), mctr, TAG_METRO_PAIR);
-        up = u24(k % 2 == 0 ? w.w0 : w.w2);
+        wp = w24(k % 2 == 0 ? w.w0 : w.w2);
         ua = u24(k % 2 == 0 ? w.w1 : w.w3);
       } else {
         w = keyed(k0, k1, (uint32_t)(x + row), mctr, TAG_METRO);
-        up = u24(w.w0);
+        wp = w24(w.w0);
         ua = u24(w.w2);
       }
-      const float t1 = fract_hw(t0 + wturn * (2.0f * up - 1.0f));
+      // 2u - 1 with u = wp 2^-24, in one exact fma
+      const float t1 = fract_hw(t0 + wturn * __builtin_fmaf(wp, 0x1.0p-23f, -1.0f));
       const float c0 = cos_t(t0), s0 = sin_t(t0), c1 = cos_t(t1), s1 = sin_t(t1);
       if (meas) measure_site(part, c0, s0, nxp, nxm, nyp, nym, nzp, nzm, jj);
       const float de = (c0 - c1) * hx + (s0 - s1) * hy;
@@ -456,9 +462,15 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
             u[2] = w.w2;
           }
           const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
-          bx_ = (px > 0.0f) & (u24(u[0]) < 1.0f - exp2_hw(m2b * px));
-          by_ = (py > 0.0f) & (u24(u[1]) < 1.0f - exp2_hw(m2b * py));
-          bz_ = (pz > 0.0f) & (u24(u[2]) < 1.0f - exp2_hw(m2b * pz));
+          // active iff p > 0 and u < 1 - exp(-2 beta p), as u 2^24 < (1 - e) 2^24
+          // with the right side in one fma
+          auto active = [&](uint32_t wu, float pb) {
+            const float e = exp2_hw(m2b * pb);
+            return (pb > 0.0f) & (w24(wu) < __builtin_fmaf(-e, 0x1.0p24f, 0x1.0p24f));
+          };
+          bx_ = active(u[0], px);
+          by_ = active(u[1], py);
+          bz_ = active(u[2], pz);
         }
         const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
         if (lane < S::RPW) {
