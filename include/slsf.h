@@ -92,6 +92,20 @@ int slsf_init_angles(slsf_batch* batch, int32_t cold);
 int slsf_get_theta(slsf_batch* batch, int32_t replica, double* theta_out);
 int slsf_set_theta(slsf_batch* batch, int32_t replica, const double* theta);
 
+/* Checkpoint image of a batch's whole run state: angles and trig cache as the
+ * kernels hold them, keyed-stream counters, acceptance and cluster tallies,
+ * and the measurement state (open-bin accumulators, bin counters, completed
+ * bins).  Replaces the serde checkpoint of ThetaModel (src/model.rs:255-271)
+ * and the RNG position of FastRng (src/types.rs:27-36): a batch created from
+ * the same replica descriptors and loaded from the image continues with
+ * exactly the sweeps, uniforms and bins the saved one would have produced.
+ * The image is versioned bytes; get its size, then save into a buffer of at
+ * least that size.  load checks the version, precision, replica count and
+ * each replica's size, seed, temperature, proposal width and wolff steps. */
+int slsf_checkpoint_size(slsf_batch* batch, int64_t* bytes);
+int slsf_checkpoint_save(slsf_batch* batch, void* buffer, int64_t bytes);
+int slsf_checkpoint_load(slsf_batch* batch, const void* buffer, int64_t bytes);
+
 /* Counters of keyed uniforms: Metropolis sweeps and Wolff clusters done so far.
  * Together with theta they are the whole Markov-chain state. */
 int slsf_get_counters(const slsf_batch* batch, uint32_t* metro_sweeps, uint32_t* wolff_steps);

@@ -831,3 +831,217 @@ int slsf_get_bins(slsf_batch* b, int32_t replica, int32_t series, double* out, i
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// checkpoint image: the whole run state of a batch as bytes
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char CKPT_MAGIC[8] = {'S', 'L', 'S', 'F', 'C', 'K', 'P', 'T'};
+constexpr uint32_t CKPT_VERSION = 1;
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  int32_t precision;
+  int32_t nrep;
+  int32_t binsize;
+  int32_t nbins;       // completed bins stored per replica and series
+  int32_t measured;    // 1: the accumulators and bin counters follow
+  int64_t total_sites;
+};
+struct CkptReplica {   // identity of a replica: a checkpoint only loads into the same one
+  int32_t lx, ly, lz, wolff_steps;
+  uint64_t seed;
+  double temperature, width;
+};
+struct CkptChain {     // chain position and run tallies
+  uint32_t metro_ctr, wolff_ctr;
+  double acc_sum;
+  int64_t acc_count;
+  double cluster_sum;
+  int64_t cluster_count;
+};
+static_assert(sizeof(CkptHeader) == 40 && sizeof(CkptReplica) == 40 && sizeof(CkptChain) == 40,
+              "checkpoint records are packed");
+
+size_t real_bytes(const slsf_batch* b) { return is_fp64(b) ? 8 : 4; }
+
+int ckpt_layout(slsf_batch* b, CkptHeader& h, size_t& bytes) {
+  const size_t nrep = b->reps.size();
+  std::memcpy(h.magic, CKPT_MAGIC, 8);
+  h.version = CKPT_VERSION;
+  h.precision = b->precision;
+  h.nrep = (int32_t)nrep;
+  h.binsize = b->binsize;
+  h.nbins = 0;
+  h.measured = b->d_accum ? 1 : 0;
+  h.total_sites = b->g.total_sites;
+  if (b->d_bin_state) {
+    std::vector<int32_t> st(nrep * 2);
+    HIP_TRY(hipMemcpy(st.data(), b->d_bin_state, sizeof(int32_t) * st.size(),
+                      hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < nrep; ++r) h.nbins = std::max(h.nbins, std::min(st[2 * r + 1], b->bin_cap));
+  }
+  bytes = sizeof(CkptHeader) + nrep * (sizeof(CkptReplica) + sizeof(CkptChain)) +
+          (size_t)h.total_sites * 3 * real_bytes(b);
+  if (h.measured)
+    bytes += nrep * SLSF_NUM_SERIES * sizeof(double) + nrep * 2 * sizeof(int32_t) +
+             nrep * SLSF_NUM_SERIES * (size_t)h.nbins * sizeof(double);
+  return SLSF_OK;
+}
+
+CkptReplica identity(const HostReplica& r) {
+  CkptReplica c;
+  c.lx = r.lx;
+  c.ly = r.ly;
+  c.lz = r.lz;
+  c.wolff_steps = r.wolff_steps;
+  c.seed = r.seed;
+  c.temperature = r.temperature;
+  c.width = r.width;
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slsf_checkpoint_size(slsf_batch* b, int64_t* bytes) {
+  clear_error();
+  if (!b || !bytes) return fail(SLSF_E_INVALID, "null argument");
+  CkptHeader h;
+  size_t n = 0;
+  if (int rc = ckpt_layout(b, h, n)) return rc;
+  *bytes = (int64_t)n;
+  return SLSF_OK;
+}
+
+int slsf_checkpoint_save(slsf_batch* b, void* buffer, int64_t bytes) {
+  clear_error();
+  if (!b || !buffer) return fail(SLSF_E_INVALID, "null argument");
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  CkptHeader h;
+  size_t need = 0;
+  if (int rc = ckpt_layout(b, h, need)) return rc;
+  if (bytes < (int64_t)need) return fail(SLSF_E_INVALID, "checkpoint buffer is too small");
+  if (int rc = sync_reps_from_device(b)) return rc;
+  const size_t nrep = b->reps.size();
+  char* p = static_cast<char*>(buffer);
+  std::memcpy(p, &h, sizeof h);
+  p += sizeof h;
+  for (size_t r = 0; r < nrep; ++r) {
+    const CkptReplica c = identity(b->reps[r]);
+    std::memcpy(p, &c, sizeof c);
+    p += sizeof c;
+  }
+  for (size_t r = 0; r < nrep; ++r) {
+    CkptChain c;
+    c.metro_ctr = b->reps_host[r].metro_ctr;
+    c.wolff_ctr = b->reps_host[r].wolff_ctr;
+    c.acc_sum = b->acc_sum[r];
+    c.acc_count = b->acc_count[r];
+    c.cluster_sum = b->cluster_sum[r];
+    c.cluster_count = b->cluster_count[r];
+    std::memcpy(p, &c, sizeof c);
+    p += sizeof c;
+  }
+  // angles and their trig cache exactly as the kernels hold them
+  const size_t tb = (size_t)h.total_sites * real_bytes(b);
+  HIP_TRY(hipMemcpy(p, b->g.theta, tb, hipMemcpyDeviceToHost));
+  p += tb;
+  HIP_TRY(hipMemcpy(p, b->g.cs, 2 * tb, hipMemcpyDeviceToHost));
+  p += 2 * tb;
+  if (h.measured) {
+    HIP_TRY(hipMemcpy(p, b->d_accum, sizeof(double) * nrep * SLSF_NUM_SERIES,
+                      hipMemcpyDeviceToHost));
+    p += sizeof(double) * nrep * SLSF_NUM_SERIES;
+    HIP_TRY(hipMemcpy(p, b->d_bin_state, sizeof(int32_t) * nrep * 2, hipMemcpyDeviceToHost));
+    p += sizeof(int32_t) * nrep * 2;
+    for (size_t k = 0; k < nrep * SLSF_NUM_SERIES && h.nbins > 0; ++k) {
+      HIP_TRY(hipMemcpy(p, b->d_bins + k * b->bin_cap, sizeof(double) * h.nbins,
+                        hipMemcpyDeviceToHost));
+      p += sizeof(double) * h.nbins;
+    }
+  }
+  return SLSF_OK;
+}
+
+int slsf_checkpoint_load(slsf_batch* b, const void* buffer, int64_t bytes) {
+  clear_error();
+  if (!b || !buffer) return fail(SLSF_E_INVALID, "null argument");
+  const size_t nrep = b->reps.size();
+  const char* p = static_cast<const char*>(buffer);
+  CkptHeader h;
+  if (bytes < (int64_t)sizeof h) return fail(SLSF_E_INVALID, "checkpoint image is truncated");
+  std::memcpy(&h, p, sizeof h);
+  p += sizeof h;
+  if (std::memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != CKPT_VERSION)
+    return fail(SLSF_E_INVALID, "not a checkpoint image of this library version");
+  if (h.precision != b->precision)
+    return fail(SLSF_E_INVALID, "checkpoint precision does not match the batch");
+  if (h.nrep != (int32_t)nrep || h.total_sites != b->g.total_sites)
+    return fail(SLSF_E_INVALID, "checkpoint replica count does not match the batch");
+  if (h.nbins < 0 || h.binsize < 0 || (h.measured != 0 && h.measured != 1))
+    return fail(SLSF_E_INVALID, "checkpoint header is corrupt");
+  const size_t tb = (size_t)h.total_sites * real_bytes(b);
+  size_t need = sizeof h + nrep * (sizeof(CkptReplica) + sizeof(CkptChain)) + 3 * tb;
+  if (h.measured)
+    need += nrep * SLSF_NUM_SERIES * sizeof(double) + nrep * 2 * sizeof(int32_t) +
+            nrep * SLSF_NUM_SERIES * (size_t)h.nbins * sizeof(double);
+  if (bytes != (int64_t)need) return fail(SLSF_E_INVALID, "checkpoint image has the wrong size");
+  for (size_t r = 0; r < nrep; ++r) {
+    CkptReplica c;
+    std::memcpy(&c, p, sizeof c);
+    p += sizeof c;
+    const CkptReplica mine = identity(b->reps[r]);
+    if (std::memcmp(&c, &mine, sizeof c) != 0)
+      return fail(SLSF_E_INVALID, "checkpoint replica " + std::to_string(r) +
+                                      " differs from the batch's (size, seed, temperature, "
+                                      "proposal width or wolff steps)");
+  }
+  std::vector<CkptChain> chain(nrep);
+  std::memcpy(chain.data(), p, sizeof(CkptChain) * nrep);
+  p += sizeof(CkptChain) * nrep;
+  HIP_TRY(hipMemcpy(b->g.theta, p, tb, hipMemcpyHostToDevice));
+  p += tb;
+  HIP_TRY(hipMemcpy(b->g.cs, p, 2 * tb, hipMemcpyHostToDevice));
+  p += 2 * tb;
+  for (size_t r = 0; r < nrep; ++r) {
+    b->reps_host[r].metro_ctr = chain[r].metro_ctr;
+    b->reps_host[r].wolff_ctr = chain[r].wolff_ctr;
+    b->acc_sum[r] = chain[r].acc_sum;
+    b->acc_count[r] = chain[r].acc_count;
+    b->cluster_sum[r] = chain[r].cluster_sum;
+    b->cluster_count[r] = chain[r].cluster_count;
+  }
+  if (int rc = push_reps_to_device(b)) return rc;
+  if (h.measured) {
+    slsf_run_options none{};  // allocates the accumulators and bin counters
+    if (int rc = ensure_bins(b, &none)) return rc;
+    HIP_TRY(hipMemcpy(b->d_accum, p, sizeof(double) * nrep * SLSF_NUM_SERIES,
+                      hipMemcpyHostToDevice));
+    p += sizeof(double) * nrep * SLSF_NUM_SERIES;
+    HIP_TRY(hipMemcpy(b->d_bin_state, p, sizeof(int32_t) * nrep * 2, hipMemcpyHostToDevice));
+    p += sizeof(int32_t) * nrep * 2;
+    if (h.nbins > b->bin_cap) {
+      double* nb = nullptr;
+      if (int rc = b->alloc(&nb, nrep * SLSF_NUM_SERIES * (size_t)h.nbins)) return rc;
+      b->d_bins = nb;
+      b->bin_cap = h.nbins;
+    }
+    for (size_t k = 0; k < nrep * SLSF_NUM_SERIES && h.nbins > 0; ++k) {
+      HIP_TRY(hipMemcpy(b->d_bins + k * b->bin_cap, p, sizeof(double) * h.nbins,
+                        hipMemcpyHostToDevice));
+      p += sizeof(double) * h.nbins;
+    }
+  } else if (b->d_accum) {  // the image has no measurement: none is open afterwards
+    HIP_TRY(hipMemset(b->d_accum, 0, sizeof(double) * nrep * SLSF_NUM_SERIES));
+    HIP_TRY(hipMemset(b->d_bin_state, 0, sizeof(int32_t) * nrep * 2));
+  }
+  b->binsize = h.binsize;
+  b->initialised = true;
+  return SLSF_OK;
+}
+
+}  // extern "C"

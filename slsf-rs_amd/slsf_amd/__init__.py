@@ -155,6 +155,9 @@ _SIGS = {
     "slsf_set_theta": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
     "slsf_get_counters": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p]),
     "slsf_set_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    "slsf_checkpoint_size": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
+    "slsf_checkpoint_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "slsf_checkpoint_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "slsf_metropolis_sweep": (ctypes.c_int, [ctypes.c_void_p, _dp, _u8p]),
     "slsf_wolff_cluster_step": (ctypes.c_int, [ctypes.c_void_p, _i64p, _u8p]),
     "slsf_sweep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp]),
@@ -316,6 +319,21 @@ class Batch:
 
     def set_counters(self, metro: int, wolff: int):
         _check(_lib.slsf_set_counters(self._h, metro, wolff))
+
+    def checkpoint(self) -> np.ndarray:
+        """The batch's whole run state as a checkpoint image (uint8 array)."""
+        n = ctypes.c_int64()
+        _check(_lib.slsf_checkpoint_size(self._h, ctypes.byref(n)))
+        img = np.empty(n.value, dtype=np.uint8)
+        _check(_lib.slsf_checkpoint_save(self._h, img.ctypes.data_as(ctypes.c_void_p), n.value))
+        return img
+
+    def restore(self, image) -> None:
+        """Loads a checkpoint image saved from a batch of the same replicas."""
+        img = np.frombuffer(image, dtype=np.uint8) if not isinstance(image, np.ndarray) \
+            else np.ascontiguousarray(image, dtype=np.uint8)
+        _check(_lib.slsf_checkpoint_load(self._h, img.ctypes.data_as(ctypes.c_void_p),
+                                         img.size))
 
     def metropolis_sweep(self, want_decisions: bool = False):
         acc = np.zeros(len(self), dtype=np.float64)

@@ -14,6 +14,8 @@
   name with the reference's messages.
 * ``dataframe_from_result`` / ``write_dataframe`` restate src/result_tools.rs:47-
   112,160-251 (columns, ``{:.16}`` floats, quoting of delimited fields).
+* ``run_job_resumable`` is run_job under the job's run_time / checkpoint_time
+  budget, resumable from a checkpoint image of the batch.
 * ``write_job_measurements`` / ``write_task_measurements_to_path`` follow
   src/results.rs:38-129: one ``task{index+1:04}/run0001.meas.h5`` per task with
   ``observables/<series>/{bin_length, samples}`` (scalar int64; float64 bin
@@ -34,6 +36,7 @@ from __future__ import annotations
 import json
 import math
 import os
+import time
 from typing import Sequence
 
 import numpy as np
@@ -72,51 +75,138 @@ def _binned(bins: np.ndarray, bin_len: int) -> dict:
     return {"mean": mean, "error": err, "internal_bin_len": int(bin_len)}
 
 
-def run_job(job: Job, rank: int = 0, world_size: int = 1, precision: int = FP32,
-            sweeps_per_launch: int = 0, use_graph: bool = False) -> dict:
-    """Runs this rank's tasks of `job` on the GPU and returns a ThetaJobResult."""
+def _rank_batch(job: Job, rank: int, world_size: int, precision: int):
     if world_size < 1 or not 0 <= rank < world_size:
         raise SlsfError(1, "rank must be below world_size")
     idx = job.selected_tasks(rank, world_size)
     tasks = [job.task(i) for i in idx]
-    result = {"job_name": job.name, "rank": rank, "world_size": world_size, "tasks": []}
     if not tasks:
-        return result
-    therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
+        return idx, tasks, None
     reps = [Replica(t["l_x"], t["l_y"], t["l_z"], t["temperature"], t["j_xy_array"],
                     t["j_z_array"], int(t["seed"]), t["proposal_width"], int(t["wolff_steps"]))
             for t in tasks]
-    b = Batch(reps, precision)
+    return idx, tasks, Batch(reps, precision)
+
+
+def _task_results(b: Batch, idx, tasks, measured: int, binsize: int) -> list:
+    """ThetaTaskResult entries of a batch that has measured `measured` sweeps."""
+    out = []
+    acceptance = b.acceptance()
+    for k, (i, t) in enumerate(zip(idx, tasks)):
+        bins = b.all_bins(k) if measured else np.zeros((len(SERIES), 0))
+        obs = {name: _binned(bins[s], binsize) for s, name in enumerate(SERIES)}
+        if bins.shape[1] > 0:
+            ev = evaluate(bins, float(t["l_x"] * t["l_y"] * t["l_z"]), 1.0 / t["temperature"])
+            for name in EVALS:
+                if name in obs:
+                    continue
+                mean, err = ev[name]
+                obs[name] = {"mean": float(mean), "error": float(err),
+                             "internal_bin_len": binsize}
+        entry = {"task": _task_json(t)}
+        if i != 0:
+            entry["task_index"] = int(i)  # serialised only when nonzero
+        entry.update({"observables": dict(sorted(obs.items())),
+                      "acceptance": float(acceptance[k]), "measurements": measured,
+                      "measurement_bins": {name: np.array(bins[s], dtype=np.float64)
+                                           for s, name in enumerate(SERIES)
+                                           if bins.shape[1] > 0}})
+        out.append(entry)
+    return out
+
+
+def run_job(job: Job, rank: int = 0, world_size: int = 1, precision: int = FP32,
+            sweeps_per_launch: int = 0, use_graph: bool = False) -> dict:
+    """Runs this rank's tasks of `job` on the GPU and returns a ThetaJobResult."""
+    idx, tasks, b = _rank_batch(job, rank, world_size, precision)
+    result = {"job_name": job.name, "rank": rank, "world_size": world_size, "tasks": []}
+    if b is None:
+        return result
+    therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
     try:
         b.init_angles()
         if therm:
             b.run(therm, 0, 1, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
         if sweeps:
             b.run(0, sweeps, binsize, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
-        acceptance = b.acceptance()
-        for k, (i, t) in enumerate(zip(idx, tasks)):
-            bins = b.all_bins(k) if sweeps else np.zeros((len(SERIES), 0))
-            obs = {name: _binned(bins[s], binsize) for s, name in enumerate(SERIES)}
-            if bins.shape[1] > 0:
-                ev = evaluate(bins, float(t["l_x"] * t["l_y"] * t["l_z"]), 1.0 / t["temperature"])
-                for name in EVALS:
-                    if name in obs:
-                        continue
-                    mean, err = ev[name]
-                    obs[name] = {"mean": float(mean), "error": float(err),
-                                 "internal_bin_len": binsize}
-            entry = {"task": _task_json(t)}
-            if i != 0:
-                entry["task_index"] = int(i)  # serialised only when nonzero
-            entry.update({"observables": dict(sorted(obs.items())),
-                          "acceptance": float(acceptance[k]), "measurements": sweeps,
-                          "measurement_bins": {name: np.array(bins[s], dtype=np.float64)
-                                               for s, name in enumerate(SERIES)
-                                               if bins.shape[1] > 0}})
-            result["tasks"].append(entry)
+        result["tasks"] = _task_results(b, idx, tasks, sweeps, binsize)
     finally:
         b.close()
     return result
+
+
+def checkpoint_path(job_name: str, rank: int, output_dir: str | None = None) -> str:
+    """Where a rank's checkpoint lives: in the job's measurement directory."""
+    return os.path.join(measurement_dir(job_name, output_dir), f"rank{rank}.checkpoint")
+
+
+def _save_image(b: Batch, path: str) -> None:
+    parent = os.path.dirname(path)
+    if parent:
+        os.makedirs(parent, exist_ok=True)
+    tmp = path + ".tmp"
+    b.checkpoint().tofile(tmp)
+    os.replace(tmp, path)
+
+
+def run_job_resumable(job: Job, path: str, rank: int = 0, world_size: int = 1,
+                      run_time: float | None = None, checkpoint_time: float | None = None,
+                      chunk_sweeps: int = 1000, precision: int = FP32) -> tuple:
+    """run_job under a wall-time budget, resumable from a checkpoint file.
+
+    The reference parses run_time / checkpoint_time (src/model.rs:567-574,
+    1033-1060) but its runner does not use them (src/results.rs:195-198); here
+    they are wired: the rank's batch resumes from `path` when that file exists
+    (a checkpoint image: angles, keyed-stream counters, tallies, open bins),
+    runs in chunks of `chunk_sweeps`, writes `path` every `checkpoint_time`
+    seconds, and stops once `run_time` seconds have passed, writing `path`
+    last.  A resumed run continues the very chain and bins of the interrupted
+    one.  Defaults come from the job (12:00:00 and 00:30:00 unless it says
+    otherwise).  Returns (ThetaJobResult of the sweeps done so far, summary
+    with the fields of JobRunSummary: task_count, elapsed_seconds,
+    checkpoint_paths, stopped_early)."""
+    started = time.monotonic()
+    if run_time is None or checkpoint_time is None:
+        rt, ct = job.durations()
+        run_time = rt if run_time is None else run_time
+        checkpoint_time = ct if checkpoint_time is None else checkpoint_time
+    if chunk_sweeps < 1:
+        raise SlsfError(1, "chunk_sweeps must be positive")
+    idx, tasks, b = _rank_batch(job, rank, world_size, precision)
+    result = {"job_name": job.name, "rank": rank, "world_size": world_size, "tasks": []}
+    summary = {"task_count": len(tasks), "elapsed_seconds": 0.0, "checkpoint_paths": [],
+               "stopped_early": False}
+    if b is None:
+        return result, summary
+    therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
+    try:
+        if os.path.exists(path):
+            b.restore(np.fromfile(path, dtype=np.uint8))
+        else:
+            b.init_angles()
+        done = int(b.counters()[0][0])
+        last = time.monotonic()
+        while done < therm + sweeps:
+            if time.monotonic() - started >= run_time:
+                summary["stopped_early"] = True
+                break
+            if done < therm:
+                n = min(chunk_sweeps, therm - done)
+                b.run(n, 0, 1)
+            else:
+                n = min(chunk_sweeps, therm + sweeps - done)
+                b.run(0, n, binsize)
+            done += n
+            if time.monotonic() - last >= checkpoint_time:
+                _save_image(b, path)
+                last = time.monotonic()
+        _save_image(b, path)
+        summary["checkpoint_paths"] = [path]
+        result["tasks"] = _task_results(b, idx, tasks, max(0, done - therm), binsize)
+    finally:
+        b.close()
+    summary["elapsed_seconds"] = time.monotonic() - started
+    return result, summary
 
 
 def _file_stem(job_name: str) -> str:
