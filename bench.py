@@ -225,7 +225,11 @@ def bench_c3(args, slsf):
         issue = kernel_rate * prof["valu_per_update"] / 64.0
         valu = {"instr_per_update": prof["valu_per_update"], "wave_instr_per_s": issue,
                 "fma_issue_measured": ceil["fma_wave_instr_per_s"],
-                "frac": issue / ceil["fma_wave_instr_per_s"]}
+                "frac": issue / ceil["fma_wave_instr_per_s"],
+                # share of the SIMDs' cycles with a vector instruction issued, from
+                # the counter run (SQ_ACTIVE_INST_VALU over SQ_WAVE_CYCLES, 4 waves
+                # per SIMD for the whole launch)
+                "simd_valu_busy": prof.get("simd_valu_busy")}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     line(value, "site-updates/s", args.steps, args.warmup, 1e3 * st["seconds"] / args.steps,
          {"workload": f"L=32 batched disorder average: {args.realizations} realizations x "
