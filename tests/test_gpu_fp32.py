@@ -302,3 +302,30 @@ def test_mixed_resident_and_hbm_sizes_follow_their_own_chains():
         one.run(6, 12, 4)
         assert np.array_equal(mix.get_theta(i), one.get_theta(0)), i
         assert np.array_equal(mix.all_bins(i), one.all_bins(0)), i
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_large_lattice_invariants(precision):
+    """One L=256 replica (16.8 M sites, 4096 union-find tiles) on the HBM
+    kernels: after thermalization and measured sweeps the angles stay in
+    [0, 2 pi) (FP32 turns) or in [0, 3 pi) (FP64 replay, which keeps the
+    reference's single-shift wrap of the Wolff reflection), the counters
+    advance, acceptance and cluster size are inside their ranges, and the bins
+    agree with a direct measurement of the final state (binsize 1)."""
+    prec = slsf.FP32 if precision == "fp32" else slsf.FP64_REPLAY
+    L = 256
+    b = slsf.Batch(reps_for(L, 1, T=1.4, seed0=31), prec)
+    b.init_angles()
+    st = b.run(2, 3, 1)
+    assert st["site_updates"] == L ** 3 * 5
+    th = b.get_theta(0)
+    top = 2 * math.pi if precision == "fp32" else 3 * math.pi
+    assert np.all(th >= 0.0) and np.all(th < top)
+    assert list(b.counters()[0]) == [5] and list(b.counters()[1]) == [5]
+    assert 0.0 < st["acceptance_mean"] < 1.0
+    assert 1.0 <= st["mean_cluster_size"] <= L ** 3
+    o = b.observables()[0]
+    e = b.all_bins(0)[0]
+    assert len(e) == 3 and np.all(np.isfinite(e))
+    tol = 2e-5 if precision == "fp32" else 1e-10
+    assert abs(e[-1] - o["energy"]) < tol * abs(o["energy"]) + 1e-6

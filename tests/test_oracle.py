@@ -288,6 +288,13 @@ def test_wolff_probability_and_reflection():
     t = L.or_wolff_reflect_angle(0.37, 1.20)
     assert abs(L.or_wolff_reflect_angle(t, 1.20) - 0.37) <= 1e-12
     assert 0 <= t < 2 * math.pi
+    # The single-shift wrap (src/types.rs:335-343) brings 2 phi + pi - theta
+    # into [0, 2 pi) only from (-2 pi, 4 pi); with phi in [0, 2 pi) the argument
+    # reaches up to 5 pi, so a reflection may leave an angle in [2 pi, 3 pi),
+    # which the next Metropolis move wraps back.  The restatement keeps this.
+    t = L.or_wolff_reflect_angle(0.1, 6.0)
+    assert t == (2.0 * 6.0 + math.pi - 0.1) - 2 * math.pi
+    assert 2 * math.pi <= t < 3 * math.pi
 
 
 @pytest.mark.parametrize("use_scratch", [True, False])
