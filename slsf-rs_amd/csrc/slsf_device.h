@@ -232,8 +232,17 @@ constexpr int TALLY_SLOTS = 64;
 // block partition of the fused observables: 2048 sites (8 per thread of a
 // 256-thread block) per block, blocks never straddle planes; a function of the
 // replica's own sizes only, so its FP64 sums do not depend on the batch
-constexpr int OBS_THREADS = 256, OBS_PER_THREAD = 8, OBS_CHUNK = OBS_THREADS * OBS_PER_THREAD;
-SLSF_HD int obs_chunks_per_plane(int lx, int ly) { return (lx * ly + OBS_CHUNK - 1) / OBS_CHUNK; }
+// A block of OBS_THREADS threads sums one chunk of a plane, obs_per_thread
+// sites per thread: 8 for planes up to 8192 sites (more blocks for the small
+// single-replica lattices, which are latency-bound), 32 above (a quarter of
+// the block reductions where the sweep streams many large planes).  The
+// partition depends on the replica's extents only.
+constexpr int OBS_THREADS = 256;
+SLSF_HD int obs_per_thread(int lx, int ly) { return lx * ly > 8192 ? 32 : 8; }
+SLSF_HD int obs_chunks_per_plane(int lx, int ly) {
+  const int chunk = OBS_THREADS * obs_per_thread(lx, ly);
+  return (lx * ly + chunk - 1) / chunk;
+}
 SLSF_HD int obs_blocks_of(int lx, int ly, int lz) { return lz * obs_chunks_per_plane(lx, ly); }
 
 struct WolffDev {  // per replica, per cluster step
