@@ -13,6 +13,7 @@
 
 #include "../../include/slsf.h"
 #include "common.h"
+#include "driver_session.h"
 #include "launch.h"
 #include "resident.h"
 
@@ -1045,3 +1046,88 @@ int slsf_checkpoint_load(slsf_batch* b, const void* buffer, int64_t bytes) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the plain driver's session: sweeps, measurements and correlation sums stay
+// on the stream until driver_close
+// ---------------------------------------------------------------------------
+namespace slsf {
+
+struct DriverSession {
+  slsf_batch* b = nullptr;
+  double* series = nullptr;   // [nmeas][8]
+  double* rows_xy = nullptr;  // [rxy][ly * lz][3]
+  double* rows_z = nullptr;   // [rz][ly * lz]
+  double* sums = nullptr;
+  int64_t nmeas = 0, nsums = 0;
+  int rxy = 0, rz = 0, wmax = 0;
+};
+
+int driver_open(slsf_batch* b, int64_t nmeas, int rxy, int rz, DriverSession** out) {
+  if (!b || !out) return fail(SLSF_E_INVALID, "null argument");
+  if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  auto s = std::make_unique<DriverSession>();
+  s->b = b;
+  s->nmeas = nmeas;
+  s->rxy = rxy;
+  s->rz = rz;
+  s->wmax = max_wolff_steps(b);
+  const auto& r = b->reps[0];
+  const int64_t rows = (int64_t)r.ly * r.lz;
+  s->nsums = 3 * (int64_t)rxy + (int64_t)r.lz * rxy + rz;
+  if (int rc = b->alloc(&s->series, (size_t)std::max<int64_t>(1, nmeas) * 8)) return rc;
+  if (int rc = b->alloc(&s->rows_xy, (size_t)std::max<int64_t>(1, rxy * rows * 3))) return rc;
+  if (int rc = b->alloc(&s->rows_z, (size_t)std::max<int64_t>(1, rz * rows))) return rc;
+  if (int rc = b->alloc(&s->sums, (size_t)std::max<int64_t>(1, s->nsums))) return rc;
+  HIP_TRY(hipMemsetAsync(s->sums, 0, sizeof(double) * (size_t)std::max<int64_t>(1, s->nsums),
+                         b->stream));
+  launch_clear_counters(b->g, b->stream);
+  launch_sweep_end(b->g, 0, 0, b->stream);
+  HIP_TRY(hipGetLastError());
+  *out = s.release();
+  return SLSF_OK;
+}
+
+int driver_sweeps(DriverSession* s, int nsweeps) {
+  for (int k = 0; k < nsweeps; ++k)
+    if (int rc = hbm_sweep(s->b, s->wmax)) return rc;
+  return SLSF_OK;
+}
+
+int driver_measure(DriverSession* s, int64_t index) {
+  if (index < 0 || index >= s->nmeas) return fail(SLSF_E_INVALID, "measurement index out of range");
+  slsf_batch* b = s->b;
+  if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
+  else launch_observables<float>(b->g, b->stream);
+  launch_fold_observables(b->g, 0, s->series + index * 8, b->stream);
+  HIP_TRY(hipGetLastError());
+  return SLSF_OK;
+}
+
+int driver_correlations(DriverSession* s) {
+  if (s->rxy + s->rz == 0) return SLSF_OK;
+  slsf_batch* b = s->b;
+  if (is_fp64(b)) launch_corr<double>(b->g, 0, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
+  else launch_corr<float>(b->g, 0, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
+  launch_fold_correlations(b->g, 0, s->rows_xy, s->rows_z, s->rxy, s->rz, s->sums, b->stream);
+  HIP_TRY(hipGetLastError());
+  return SLSF_OK;
+}
+
+int driver_close(DriverSession* s, int64_t nsweeps, double* acceptance_sum, double* series,
+                 double* sums) {
+  slsf_batch* b = s->b;
+  if (series && s->nmeas > 0)
+    HIP_TRY(hipMemcpyAsync(series, s->series, sizeof(double) * (size_t)s->nmeas * 8,
+                           hipMemcpyDeviceToHost, b->stream));
+  if (sums && s->nsums > 0)
+    HIP_TRY(hipMemcpyAsync(sums, s->sums, sizeof(double) * (size_t)s->nsums,
+                           hipMemcpyDeviceToHost, b->stream));
+  if (int rc = gather_acceptance(b, nsweeps, acceptance_sum)) return rc;  // synchronises
+  return gather_clusters(b, clusters_in(b, nsweeps), nullptr);
+}
+
+void driver_free(DriverSession* s) { delete s; }
+int64_t driver_sums_size(const DriverSession* s) { return s->nsums; }
+
+}  // namespace slsf

@@ -10,6 +10,7 @@
 
 #include "../../include/slsf.h"
 #include "common.h"
+#include "driver_session.h"
 
 using namespace slsf;
 
@@ -40,7 +41,11 @@ double mean_square(const std::vector<double>& x) {
 
 struct BatchGuard {
   slsf_batch* b = nullptr;
-  ~BatchGuard() { slsf_batch_destroy(b); }
+  DriverSession* s = nullptr;
+  ~BatchGuard() {
+    driver_free(s);
+    slsf_batch_destroy(b);
+  }
 };
 
 }  // namespace
@@ -97,34 +102,44 @@ extern "C" int slsf_run_theta_simulation(int32_t l_x, int32_t l_y, int32_t l_z, 
   if (int rc = slsf_correlation_rmax(g.b, 0, o->correlation_rmax, o->correlation_rmax_xy,
                                      o->correlation_rmax_z, &rxy, &rz))
     return rc;
-  double acc_sum = 0.0;
+  // The whole run is queued on the batch's stream: measurements go to device
+  // buffers (the host folds' order, on the device) and are read back once.
+  const bool corr = rxy > 0 || rz > 0;
+  if (int rc = driver_open(g.b, nmeas, rxy, rz, &g.s)) return rc;
   if (o->thermal_sweeps > 0)
-    if (int rc = slsf_sweep(g.b, o->thermal_sweeps, &acc_sum)) return rc;
-  std::vector<double> en, cx, cy, cz, sx, sy, sz, m2;
-  const size_t nz = (size_t)l_z;
-  std::vector<double> cx_sum(rxy), cy_sum(rxy), cxy_sum(rxy), cbz_sum(nz * rxy), cz_sum(rz);
-  std::vector<double> tx(rxy + 1), ty(rxy + 1), txy(rxy + 1), tbz(nz * rxy + 1), tz(rz + 1);
+    if (int rc = driver_sweeps(g.s, o->thermal_sweeps)) return rc;
   int ncorr = 0;
+  int64_t idx = 0;
   for (int step = 1; step <= o->measure_sweeps; ++step) {
-    if (int rc = slsf_sweep(g.b, 1, &acc_sum)) return rc;
+    if (int rc = driver_sweeps(g.s, 1)) return rc;
     if (step % o->measure_interval != 0) continue;
-    slsf_observables ob;
-    if (int rc = slsf_measure_observables(g.b, &ob)) return rc;
-    en.push_back(ob.energy);
-    cx.push_back(ob.cos_x); cy.push_back(ob.cos_y); cz.push_back(ob.cos_z);
-    sx.push_back(ob.sin_x); sy.push_back(ob.sin_y); sz.push_back(ob.sin_z);
-    m2.push_back(ob.magnetization_squared);
-    const int idx = (int)en.size();
-    if ((rxy > 0 || rz > 0) && (idx - 1) % o->correlation_interval == 0) {
-      if (int rc = slsf_measure_correlations(g.b, 0, -1, rxy, rz, tx.data(), ty.data(),
-                                             txy.data(), tbz.data(), tz.data()))
-        return rc;
-      for (int i = 0; i < rxy; ++i) { cx_sum[i] += tx[i]; cy_sum[i] += ty[i]; cxy_sum[i] += txy[i]; }
-      for (size_t i = 0; i < nz * rxy; ++i) cbz_sum[i] += tbz[i];
-      for (int i = 0; i < rz; ++i) cz_sum[i] += tz[i];
+    if (int rc = driver_measure(g.s, idx)) return rc;
+    ++idx;
+    if (corr && (idx - 1) % o->correlation_interval == 0) {
+      if (int rc = driver_correlations(g.s)) return rc;
       ++ncorr;
     }
   }
+  std::vector<double> series((size_t)nmeas * 8), sums((size_t)driver_sums_size(g.s));
+  double acc_sum = 0.0;
+  if (int rc = driver_close(g.s, (int64_t)o->thermal_sweeps + o->measure_sweeps, &acc_sum,
+                            series.data(), sums.data()))
+    return rc;
+  std::vector<double> en(nmeas), cx(nmeas), cy(nmeas), cz(nmeas), sx(nmeas), sy(nmeas),
+      sz(nmeas), m2(nmeas);
+  for (int i = 0; i < nmeas; ++i) {
+    const double* v = &series[(size_t)i * 8];
+    en[i] = v[0];
+    m2[i] = v[1];
+    cx[i] = v[2]; cy[i] = v[3]; cz[i] = v[4];
+    sx[i] = v[5]; sy[i] = v[6]; sz[i] = v[7];
+  }
+  const size_t nz = (size_t)l_z;
+  const double* cx_sum = sums.data();
+  const double* cy_sum = cx_sum + rxy;
+  const double* cxy_sum = cy_sum + rxy;
+  const double* cbz_sum = cxy_sum + rxy;
+  const double* cz_sum = cbz_sum + nz * rxy;
   // estimators (src/simulation.rs:136-194)
   const double vol = (double)l_x * l_y * l_z, beta = 1.0 / temperature;
   const double msx = mean(sx), msy = mean(sy), msz = mean(sz);

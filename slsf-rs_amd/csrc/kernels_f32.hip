@@ -106,4 +106,75 @@ void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_
                                      bin_cap);
 }
 
+// ---------------------------------------------------------------------------
+// device folds of the plain driver: the same sequential FP64 sums the host
+// folds of slsf_measure_observables / slsf_measure_correlations do, so a run
+// that never reads back until its end gets identical numbers
+// ---------------------------------------------------------------------------
+__global__ void k_fold_observables(const RepDev* __restrict__ rep, const double* __restrict__ part,
+                                   double* __restrict__ out) {
+  const RepDev d = *rep;
+  const int nb = obs_blocks_of(d.lx, d.ly, d.lz);
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 9; ++k) v[k] += part[(int64_t)b * 9 + k];
+  const double vol = (double)d.volume;
+  out[0] = v[0] / vol;
+  out[1] = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
+  for (int k = 3; k < 9; ++k) out[k - 1] = v[k];
+}
+
+// sums[] = {corr_x[rxy], corr_y[rxy], corr_xy[rxy], corr_xy_by_z[lz * rxy], corr_z[rz]}
+// += this measurement's values; thread r - 1 < rxy takes shift r in x/y,
+// thread rxy + r - 1 shift r in z (src/observables.rs:189-252 order)
+__global__ void k_fold_correlations(const RepDev* __restrict__ rep,
+                                    const double* __restrict__ hxy, const double* __restrict__ hz,
+                                    int rxy, int rz, double* __restrict__ sums) {
+  const RepDev d = *rep;
+  const int task = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t rows = (int64_t)d.ly * d.lz;
+  const double vol = (double)d.volume, area = (double)d.lx * d.ly;
+  double* cx = sums;
+  double* cy = cx + rxy;
+  double* cxy = cy + rxy;
+  double* cbz = cxy + rxy;
+  double* cz = cbz + (int64_t)d.lz * rxy;
+  if (task < rxy) {
+    const int rr = task + 1;
+    double sx = 0, sy = 0;
+    for (int z = 0; z < d.lz; ++z) {
+      double lsx = 0, lsy = 0;
+      for (int y = 0; y < d.ly; ++y) {
+        const double* o = hxy + ((int64_t)(rr - 1) * rows + (int64_t)z * d.ly + y) * 3;
+        lsx += o[0];
+        lsx += o[1];
+        lsy += o[2];
+      }
+      sx += lsx;
+      sy += lsy;
+      cbz[(int64_t)z * rxy + (rr - 1)] += (lsx + lsy) / (2.0 * area);
+    }
+    cx[rr - 1] += sx / vol;
+    cy[rr - 1] += sy / vol;
+    cxy[rr - 1] += (sx / vol + sy / vol) / 2.0;
+  } else if (task < rxy + rz) {
+    const int rr = task - rxy + 1;
+    double sz = 0;
+    for (int64_t k = 0; k < rows; ++k) sz += hz[(int64_t)(rr - 1) * rows + k];
+    cz[rr - 1] += sz / vol;
+  }
+}
+
+void launch_fold_observables(const GlobalState& g, int replica, double* out, hipStream_t s) {
+  k_fold_observables<<<1, 1, 0, s>>>(g.reps + replica,
+                                     g.partials + (int64_t)replica * g.obs_blocks * 9, out);
+}
+
+void launch_fold_correlations(const GlobalState& g, int replica, const double* hxy,
+                              const double* hz, int rxy, int rz, double* sums, hipStream_t s) {
+  if (rxy + rz > 0)
+    k_fold_correlations<<<(rxy + rz + 63) / 64, 64, 0, s>>>(g.reps + replica, hxy, hz, rxy, rz,
+                                                            sums);
+}
+
 }  // namespace slsf

@@ -58,6 +58,13 @@ void launch_clear_counters(const GlobalState& g, hipStream_t s);
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
                        int32_t bin_cap, hipStream_t s);
 
+// folds of one replica's observables (after launch_observables: out[8] =
+// {energy, m^2, cos_x, cos_y, cos_z, sin_x, sin_y, sin_z}) and correlation
+// rows (after launch_corr: added into sums) in the host folds' order
+void launch_fold_observables(const GlobalState& g, int replica, double* out, hipStream_t s);
+void launch_fold_correlations(const GlobalState& g, int replica, const double* hxy,
+                              const double* hz, int rxy, int rz, double* sums, hipStream_t s);
+
 // device bins of a measured run; accum == nullptr runs without measurement
 struct BinTarget {
   double* accum = nullptr;
