@@ -143,6 +143,29 @@ SLSF_D float opq(float v) {
 // a power of two stays exact, which saves the multiply and keeps every value.
 SLSF_D float w24(uint32_t w) { return (float)(w >> 8); }
 
+// Bond uniforms of the production stream: G sites of a column take their 3G
+// uniforms as consecutive 24-bit fields (most significant bits first) of the
+// words of NC keyed calls, w0..w3 of call 0, then call 1, ...: 16 sites from
+// 9 calls where L allows, 8 from 5, 4 from 3.
+template <int L>
+struct BondGroup {
+  static constexpr int G = (L % 16 == 0) ? 16 : ((L % 8 == 0) ? 8 : 4);
+  static constexpr int NC = (G * 72 + 127) / 128;
+  // the first site whose uniforms reach into call c's first word (bit 128 c):
+  // field f ends at bit 24 f + 23, so f >= floor(16 c / 3)
+  static constexpr int first_site(int c) { return (16 * c / 3) / 3; }
+};
+// 24-bit field f of the words of NC calls (f compile-time after unrolling)
+template <int NC>
+SLSF_D uint32_t field24(const Words (&w)[NC], int f) {
+  const int bit = 24 * f, wi = bit >> 5, off = bit & 31;
+  const uint32_t a = word(w[wi >> 2], wi & 3);
+  if (off == 0) return a >> 8;
+  if (off == 8) return a & 0xFFFFFFu;
+  const uint32_t b = word(w[(wi + 1) >> 2], (wi + 1) & 3);
+  return __builtin_amdgcn_alignbit(a, b, (uint32_t)(off == 16 ? 24 : 16)) & 0xFFFFFFu;
+}
+
 // hardware transcendentals: v_sin/v_cos take revolutions, v_exp is 2^x
 SLSF_D float sin_t(float t) { return __builtin_amdgcn_sinf(t); }
 SLSF_D float cos_t(float t) { return __builtin_amdgcn_cosf(t); }
@@ -423,24 +446,28 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const int wave = t >> 6, lane = t & 63;
     const float m2b = -2.0f * beta_l2e;
     const int xp = (x + 1 == L) ? 0 : x + 1;
-    // bond decisions read r from LDS only, so this loop stays rolled; it takes
-    // four sites (z .. z+3) of a column per step.  Production stream: the 12
-    // bond uniforms of the four sites are the words of three keyed calls (tags
-    // WBOND_QUAD + 0..2, index = the first site); replay stream: one call per
-    // site (tag WBOND, words 0..2), as in the FP64 path.
+    // Bond decisions read r from LDS only, so this loop stays rolled; it takes
+    // G sites (z .. z+G-1) of a column per step.  Production stream: the 3G
+    // bond uniforms of the G sites are consecutive 24-bit fields of the words
+    // of BondGroup<G>::NC keyed calls (tags WBOND_GROUP + c, index = the first
+    // site), each call made just before the first site that reaches into it;
+    // replay stream: one call per site (tag WBOND, words 0..2), as in the
+    // FP64 path.
+    using BG = BondGroup<L>;
 #pragma unroll 1
-    for (int jq = 0; jq < CPT * L; jq += 4) {
+    for (int jq = 0; jq < CPT * L; jq += BG::G) {
       const int j = jq / L, zq = jq % L;
       const int yj = y + j * S::YS;
       const int yp = (yj + 1 == L) ? 0 : yj + 1;
-      Words wq[3] = {};
-      if (PAIRED && own) {
-        const uint32_t i0 = (uint32_t)(x + L * (yj + L * zq));
+      const uint32_t i0 = (uint32_t)(x + L * (yj + L * zq));
+      Words wq[BG::NC];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) wq[c] = keyed(k0, k1, i0, wctr, TAG_WBOND_QUAD + c);
-      }
+      for (int q = 0; q < BG::G; ++q) {
+        if (PAIRED && own) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+          for (int c = 0; c < BG::NC; ++c)
+            if (BG::first_site(c) == q) wq[c] = keyed(k0, k1, i0, wctr, TAG_WBOND_GROUP + c);
+        }
         const int z = zq + q;
         const int zn = (z + 1 == L) ? 0 : z + 1;
         bool bx_ = false, by_ = false, bz_ = false;
@@ -451,26 +478,26 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           const float ryp = rbuf[x + L * (yp + L * z)];
           const float rzp = rbuf[x + L * (yj + L * zn)];
           const float4 jj = jt[z];
-          uint32_t u[3];
+          float wv[3];  // the bond uniforms' 24-bit integers as floats
           if (PAIRED) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) u[a] = word(wq[(3 * q + a) / 4], (3 * q + a) % 4);
+            for (int a = 0; a < 3; ++a) wv[a] = (float)field24<BG::NC>(wq, 3 * q + a);
           } else {
             const Words w = keyed(k0, k1, (uint32_t)i, wctr, TAG_WBOND);
-            u[0] = w.w0;
-            u[1] = w.w1;
-            u[2] = w.w2;
+            wv[0] = w24(w.w0);
+            wv[1] = w24(w.w1);
+            wv[2] = w24(w.w2);
           }
           const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
           // active iff p > 0 and u < 1 - exp(-2 beta p), as u 2^24 < (1 - e) 2^24
           // with the right side in one fma
-          auto active = [&](uint32_t wu, float pb) {
+          auto active = [&](float wu, float pb) {
             const float e = exp2_hw(m2b * pb);
-            return (pb > 0.0f) & (w24(wu) < __builtin_fmaf(-e, 0x1.0p24f, 0x1.0p24f));
+            return (pb > 0.0f) & (wu < __builtin_fmaf(-e, 0x1.0p24f, 0x1.0p24f));
           };
-          bx_ = active(u[0], px);
-          by_ = active(u[1], py);
-          bz_ = active(u[2], pz);
+          bx_ = active(wv[0], px);
+          by_ = active(wv[1], py);
+          bz_ = active(wv[2], pz);
         }
         const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
         if (lane < S::RPW) {

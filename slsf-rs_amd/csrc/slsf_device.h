@@ -19,12 +19,15 @@
 //      tag METRO_PAIR (FP32 resident kernel, production stream): index = the
 //        first of two same-colour sites z, z+2 of a column, counter = sweep
 //                                -> (proposal, accept) = (w0, w1) and (w2, w3)
-//      tags WBOND_QUAD + c, c = 0..2 (FP32 resident kernel, production stream):
-//        index = site (x, y, z) with z % 4 == 0, counter = cluster; the 12 words
-//        of the three calls, in order, are the +x, +y, +z bond uniforms of the
-//        sites z, z+1, z+2, z+3 of the column
+//      tags WBOND_GROUP + c, c < NC (FP32 resident kernel, production stream):
+//        index = site (x, y, z) with z % G == 0, counter = cluster, G = 16, 8
+//        or 4 sites of a column (the largest that divides L) and NC = 9, 5 or 3
+//        calls; the words of the calls, in order, read as consecutive 24-bit
+//        fields (most significant bits first), are the +x, +y, +z bond
+//        uniforms of the sites z .. z+G-1
 //    FP64 uniforms use 53 bits (two words), FP32 uniforms the top 24 bits of the
-//    first word of the pair; bond uniforms are w*2^-32 (FP64) or (w>>8)*2^-24.
+//    first word of the pair; bond uniforms are w*2^-32 (FP64) or (w>>8)*2^-24
+//    (a packed field f: f*2^-24).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,7 +42,7 @@ constexpr uint32_t TAG_METRO = 2;
 constexpr uint32_t TAG_WSEED = 3;
 constexpr uint32_t TAG_WBOND = 4;
 constexpr uint32_t TAG_METRO_PAIR = 5;
-constexpr uint32_t TAG_WBOND_QUAD = 6;  // and 7, 8: three calls per four sites
+constexpr uint32_t TAG_WBOND_GROUP = 16;  // .. 24: up to nine calls per sixteen sites
 
 constexpr double TWO_PI = 3.141592653589793 * 2.0;  // src/types.rs:6
 constexpr double PI = 3.141592653589793;
