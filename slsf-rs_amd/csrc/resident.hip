@@ -425,6 +425,10 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
 
   auto wolff = [&]() __attribute__((always_inline)) {
     asm volatile("" : "+v"(x), "+v"(y));
+    // the thread index made opaque too: the row loops below then compute their
+    // addresses here instead of keeping (or spilling) them across the sweep loop
+    int tw = t;
+    asm volatile("" : "+v"(tw));
     const Words ws = keyed(k0, k1, 0u, wctr, TAG_WSEED);
     const float phit = u24(ws.w0);  // phi in turns
     const int seed = (int)(((uint64_t)ws.w2 * (uint64_t)S::V) >> 32);
@@ -443,7 +447,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           }
     }
     __syncthreads();
-    const int wave = t >> 6, lane = t & 63;
+    const int wave = tw >> 6, lane = tw & 63;
     const float m2b = -2.0f * beta_l2e;
     const int xp = (x + 1 == L) ? 0 : x + 1;
     // Bond decisions read r from LDS only, so this loop stays rolled; it takes
@@ -509,7 +513,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         }
       }
     }
-    for (int r = t; r < S::ROWS; r += S::NT) vis[r] = (r == seed / L) ? (1u << (seed % L)) : 0u;
+    for (int r = tw; r < S::ROWS; r += S::NT) vis[r] = (r == seed / L) ? (1u << (seed % L)) : 0u;
     __syncthreads();
     // Straight-line path masks, in the exchange buffer (free during the fill):
     // bit x of py2[r] says the y-bonds of column x from row r to row r + 2 (same
@@ -520,13 +524,13 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     uint32_t* pz2 = py2 + S::ROWS;
     uint32_t* py4 = pz2 + S::ROWS;
     uint32_t* pz4 = py4 + S::ROWS;
-    for (int r = t; r < S::ROWS; r += S::NT) {
+    for (int r = tw; r < S::ROWS; r += S::NT) {
       const int ry = r % L, rz = r / L;
       py2[r] = bym[r] & bym[((ry + 1) % L) + L * rz];
       pz2[r] = bzm[r] & bzm[ry + L * ((rz + 1) % L)];
     }
     __syncthreads();
-    for (int r = t; r < S::ROWS; r += S::NT) {
+    for (int r = tw; r < S::ROWS; r += S::NT) {
       const int ry = r % L, rz = r / L;
       py4[r] = py2[r] & py2[((ry + 2) % L) + L * rz];
       pz4[r] = pz2[r] & pz2[ry + L * ((rz + 2) % L)];
@@ -544,7 +548,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     uint32_t nb_mask[RPT][NB], mine[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
-      const int r = t + i * S::NT;
+      const int r = tw + i * S::NT;
       const bool valid = r < S::ROWS;
       const int rr = valid ? r : 0;
       const int ry = rr % L, rz = rr / L;
@@ -571,7 +575,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       bool changed = false;
 #pragma unroll
       for (int i = 0; i < RPT; ++i) {
-        const int r = t + i * S::NT;
+        const int r = tw + i * S::NT;
         if (RPT * S::NT > S::ROWS && r >= S::ROWS) break;
         uint32_t n = mine[i];
 #pragma unroll
@@ -606,7 +610,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           }
     }
     int members = 0;
-    for (int r = t; r < S::ROWS; r += S::NT) members += __popc(vis[r]);
+    for (int r = tw; r < S::ROWS; r += S::NT) members += __popc(vis[r]);
     ncluster += members;
     wctr += 1u;
   };
