@@ -94,21 +94,27 @@ def cpu_baseline(budget_s=12.0):
     Wolff DFS with trig cache, libm sincos/exp) on one L=32 replica per core."""
     import multiprocessing as mp
     cores = min(os.cpu_count() or 1, 16)
-    rate1 = _cpu_worker((0, 3))  # calibrate one core
+    rate1 = _cpu_worker((0, 3))[0]  # calibrate one core
     sweeps = max(2, int(budget_s * rate1 / (32 ** 3)))
     t0 = time.perf_counter()
     with mp.get_context("fork").Pool(cores) as pool:
-        rates = pool.map(_cpu_worker, [(i, sweeps) for i in range(cores)])
+        out = pool.map(_cpu_worker, [(i, sweeps) for i in range(cores)])
     wall = time.perf_counter() - t0
     total = cores * sweeps * 32 ** 3
+    therm = sweeps // 5
     return {"value": total / wall, "unit": "site-updates/s", "cores": cores,
             "kind": "port",
-            "sample": f"{cores} independent L=32 replicas (T=1.3, one per core), "
-                      f"{sweeps} Metropolis+Wolff sweeps each, FP64, reference draw order",
-            "per_core": float(np.mean(rates))}
+            "sample": f"{cores} independent L=32 replicas (realizations 0..{cores - 1} of the "
+                      f"batched workload at T=1.3, one per core), {sweeps} Metropolis+Wolff "
+                      f"sweeps each ({therm} thermalization, then measured every sweep), FP64, "
+                      "reference draw order",
+            "per_core": float(np.mean([o[0] for o in out])),
+            "bins": {i: o[1] for i, o in enumerate(out)}}
 
 
 def _cpu_worker(arg):
+    """One replica: `sweeps` reference-mode sweeps, the last 4/5 measured every
+    sweep (the 13 series, binned in bins of up to 100 measurements)."""
     idx, sweeps = arg
     import ctypes
     import oracle_lib as orc
@@ -119,11 +125,52 @@ def _cpu_worker(arg):
     L.or_rng_seed(ctypes.byref(r), 7 + idx)
     L.or_initialize_angles(lat.ref, 0, ctypes.byref(r))
     lat.refresh(False)
+    therm = sweeps // 5
+    rows = []
     t0 = time.perf_counter()
-    for _ in range(sweeps):
+    for k in range(sweeps):
         lat.metro_ref(1.3, np.pi, r, True)
         lat.wolff_ref(1.3, r, True)
-    return sweeps * 32 ** 3 / (time.perf_counter() - t0)
+        if k >= therm:
+            o = lat.observables()
+            e, m2 = o["energy"], o["magnetization_squared"]
+            c = [o["cos_x"], o["cos_y"], o["cos_z"]]
+            sn = [o["sin_x"], o["sin_y"], o["sin_z"]]
+            rows.append([e, e * e, m2, m2 * m2] + c + sn + [v * v for v in sn])
+    rate = sweeps * 32 ** 3 / (time.perf_counter() - t0)
+    series = np.array(rows).T  # 13 x measurements, in the order of the GPU series
+    binsize = max(1, min(100, series.shape[1] // 20))
+    nb = series.shape[1] // binsize
+    bins = series[:, :nb * binsize].reshape(13, nb, binsize).mean(axis=2)
+    return rate, bins
+
+
+def fp64_agreement(slsf, gpu_bins, cpu_bins, volume, beta):
+    """FP32 production bins of the batched run against the FP64 CPU runs of
+    the same couplings and temperature (independent chains): per observable
+    the z-scores (gpu - cpu) / sqrt(err_gpu^2 + err_cpu^2) over the replicas,
+    and the disorder average of the differences with its combined error."""
+    names = ["Energy", "RhoXY", "RhoZ", "Binder", "Chi"]
+    out = {"replicas": len(cpu_bins), "reference": "oracle reference mode, FP64, CPU",
+           "observables": {}}
+    for name in names:
+        z, d, var = [], [], 0.0
+        for i, cb in cpu_bins.items():
+            g = slsf.evaluate(gpu_bins[i], volume, beta)[name]
+            c = slsf.evaluate(cb, volume, beta)[name]
+            err = math.hypot(g[1], c[1])
+            z.append((g[0] - c[0]) / err if err > 0 else 0.0)
+            d.append(g[0] - c[0])
+            var += err * err
+        n = len(z)
+        z = np.array(z)
+        out["observables"][name] = {
+            "max_abs_z": float(np.max(np.abs(z))),
+            "within_2_sigma": float(np.mean(np.abs(z) <= 2.0)),
+            "mean_z_squared": float(np.mean(z * z)),  # 1 +- sqrt(2 / n) when they agree
+            "mean_difference": float(np.mean(d)),
+            "mean_difference_error": math.sqrt(var) / n}
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -173,6 +220,9 @@ def bench_c3(args, slsf):
     b.init_angles()
     b.run(args.warmup, 0, 1, sweeps_per_launch=args.sweeps_per_launch)
     st = b.run(0, args.steps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
+    # bins of realizations 0..15 at T = 1.30 (index 4 of 8), for the comparison
+    # with the FP64 CPU runs of the same couplings
+    gpu_bins = {r: b.all_bins(8 * r + 4) for r in range(min(16, args.realizations))}
     b.close()
     updates = len(reps) * V * args.steps
     value = updates / st["seconds"]
@@ -231,6 +281,9 @@ def bench_c3(args, slsf):
                 # per SIMD for the whole launch)
                 "simd_valu_busy": prof.get("simd_valu_busy")}
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    agreement = None
+    if cpu is not None:
+        agreement = fp64_agreement(slsf, gpu_bins, cpu.pop("bins"), V, 1.0 / 1.3)
     line(value, "site-updates/s", args.steps, args.warmup, 1e3 * st["seconds"] / args.steps,
          {"workload": f"L=32 batched disorder average: {args.realizations} realizations x "
                       "8 T in [1.10,1.45], Metropolis + 1 Wolff + 13-series measurement "
@@ -242,7 +295,7 @@ def bench_c3(args, slsf):
          metropolis_only_rate=metro_rate, wolff_only_rate=wolff_rate,
          acceptance=st["acceptance_mean"], mean_cluster_size=st["mean_cluster_size"],
          roofline=roofline, lds_roofline=lds, valu_issue=valu, ceilings=ceil,
-         cpu_baseline=cpu,
+         cpu_baseline=cpu, fp64_cpu_agreement=agreement,
          kernel={"name": "k_resident<32, 1, paired>", "launches": st["launches"],
                  "avg_launch_ms": avg_launch_ms, "sweeps_per_launch": spl,
                  "algorithmic_bytes_per_launch": alg_bytes_launch})

@@ -178,6 +178,56 @@ def test_fp32_statistics_match_fp64_replay():
         assert abs(a.mean() - c.mean()) < 4 * se, name
 
 
+def test_fp32_evaluables_match_fp64_replay_at_l32():
+    """The headline lattice, L=32 at T=1.3 with the batched workload's
+    couplings (realizations 0..63): FP32 production runs (resident kernel)
+    and FP64 replay runs (HBM kernels, the oracle's arithmetic), independent
+    streams, 1000 + 5000 sweeps each.  Per replica the jackknife estimates of
+    the energy, helicity moduli, Binder ratio and susceptibility agree within
+    2 combined sigma for at least 85% of the replicas, the mean squared z-score
+    is near 1, and the disorder averages agree within 4 combined sigma."""
+    L, T, n = 32, 1.3, 64
+    reps = [slsf.Replica(L, L, L, T, _splitmix_layers(L, 1000 + r), np.ones(L), 77 + r)
+            for r in range(n)]
+    out = {}
+    for prec in (slsf.FP32, slsf.FP64_REPLAY):
+        b = slsf.Batch(reps, prec)
+        b.init_angles()
+        b.run(1000, 0, 1)
+        b.run(0, 5000, 100)
+        out[prec] = [b.all_bins(r) for r in range(n)]
+    V, beta = L ** 3, 1.0 / T
+    for name in ("Energy", "RhoXY", "RhoZ", "Binder", "Chi"):
+        z, d, var = [], [], 0.0
+        for r in range(n):
+            g = slsf.evaluate(out[slsf.FP32][r], V, beta)[name]
+            c = slsf.evaluate(out[slsf.FP64_REPLAY][r], V, beta)[name]
+            e = math.hypot(g[1], c[1])
+            z.append((g[0] - c[0]) / e)
+            d.append(g[0] - c[0])
+            var += e * e
+        z = np.array(z)
+        avg, err = float(np.mean(d)), math.sqrt(var) / n
+        print(f"{name}: within 2 sigma {np.mean(np.abs(z) <= 2):.3f}, mean z^2 "
+              f"{np.mean(z * z):.2f}, average difference {avg:.3e} +- {err:.3e}")
+        assert np.mean(np.abs(z) <= 2.0) >= 0.85, name
+        assert 0.4 < np.mean(z * z) < 2.0, name
+        assert abs(avg) < 4 * err, name
+
+
+def _splitmix_layers(lz, seed, lo=0.6, hi=1.8):
+    """U[lo, hi] per layer from SplitMix64(seed), the batched workload's rule."""
+    out, state = [], seed & (2 ** 64 - 1)
+    for _ in range(lz):
+        state = (state + 0x9E3779B97F4A7C15) & (2 ** 64 - 1)
+        x = state
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & (2 ** 64 - 1)
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & (2 ** 64 - 1)
+        x ^= x >> 31
+        out.append(lo + (hi - lo) * ((x >> 11) * 2.0 ** -53))
+    return np.array(out)
+
+
 def test_launch_split_invariance():
     """The same run split into launches of 7 or 64 sweeps is bit-identical:
     the FP32 state (angles in turns) is stored exactly between launches."""
