@@ -215,10 +215,12 @@ SLSF_D void measure_site(MeasPart& m, float c, float s, float2 nxp, float2 nxm, 
   m.my += s + nzp.y;
 }
 
-// PAIRED: one Philox call serves two same-colour sites of a column (production
-// stream, tag METRO_PAIR, counter index = first site of the pair); otherwise
-// one call per site with the replay path's counters (tag METRO), which lets an
-// FP32 run consume exactly the uniforms the FP64 replay path uses.
+// PAIRED (production stream): the (proposal, accept) uniforms of up to eight
+// same-colour sites of a column are consecutive 24-bit fields of the words of
+// up to three keyed calls (tags METRO_GROUP + c, counter index = the group's
+// first site); otherwise one call per site with the replay path's counters
+// (tag METRO), which lets an FP32 run consume exactly the uniforms the FP64
+// replay path uses.
 template <int L, int CPT, bool PAIRED>
 __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   using S = RShape<L, CPT>;
@@ -304,6 +306,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const int xp = (x + 1 == L) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
     const float4* jz0 = jt + z0;
     Words w{0u, 0u, 0u, 0u};
+    Words wm[3];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
     const int y = y0 + j * S::YS;
@@ -333,9 +336,17 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       const float hy = jj.x * (nxp.y + nxm.y + nyp.y + nym.y) + jj.y * nzp.y + jj.z * nzm.y;
       float wp, ua;  // wp: the proposal's 24-bit integer as a float
       if (PAIRED) {
-        if (k % 2 == 0) w = keyed(k0, k1, (uint32_t)(x + row), mctr, TAG_METRO_PAIR);
-        wp = w24(k % 2 == 0 ? w.w0 : w.w2);
-        ua = u24(k % 2 == 0 ? w.w1 : w.w3);
+        // group of sites k0g .. k0g + 7 (fewer at the column's end)
+        const int m = k % 8, k0g = k - m;  // compile-time once the loop is unrolled
+        const int n = (L / 2 - k0g) < 8 ? (L / 2 - k0g) : 8;
+        const int nc = (48 * n + 127) / 128;
+        const int grow = L * (y + L * (2 * k0g + z0));
+#pragma unroll
+        for (int c = 0; c < nc; ++c)  // each call just before the first site it serves
+          if ((16 * c / 3) / 2 == m) wm[c] = keyed(k0, k1, (uint32_t)(x + grow), mctr,
+                                                   TAG_METRO_GROUP + c);
+        wp = (float)field24<3>(wm, 2 * m);
+        ua = (float)field24<3>(wm, 2 * m + 1) * 0x1.0p-24f;
       } else {
         w = keyed(k0, k1, (uint32_t)(x + row), mctr, TAG_METRO);
         wp = w24(w.w0);

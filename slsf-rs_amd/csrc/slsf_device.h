@@ -16,9 +16,11 @@
 //      tag WSEED: index = 0,    counter = cluster    -> phi u(w0,w1), seed site (w2*V)>>32
 //      tag WBOND: index = base site of bond (base, base+e_a), counter = cluster
 //                                                    -> bond uniform from word a
-//      tag METRO_PAIR (FP32 resident kernel, production stream): index = the
-//        first of two same-colour sites z, z+2 of a column, counter = sweep
-//                                -> (proposal, accept) = (w0, w1) and (w2, w3)
+//      tags METRO_GROUP + c, c < 3 (FP32 resident kernel, production stream):
+//        index = the first of up to eight same-colour sites z, z+2, ... of a
+//        column (site k of the colour's L/2 with k % 8 == 0), counter = sweep;
+//        the words of the calls (three for eight sites, two for four), read as
+//        consecutive 24-bit fields, are (proposal, accept) of each site in turn
 //      tags WBOND_GROUP + c, c < NC (FP32 resident kernel, production stream):
 //        index = site (x, y, z) with z % G == 0, counter = cluster, G = 16, 8
 //        or 4 sites of a column (the largest that divides L) and NC = 9, 5 or 3
@@ -41,8 +43,8 @@ constexpr uint32_t TAG_INIT = 1;
 constexpr uint32_t TAG_METRO = 2;
 constexpr uint32_t TAG_WSEED = 3;
 constexpr uint32_t TAG_WBOND = 4;
-constexpr uint32_t TAG_METRO_PAIR = 5;
 constexpr uint32_t TAG_WBOND_GROUP = 16;  // .. 24: up to nine calls per sixteen sites
+constexpr uint32_t TAG_METRO_GROUP = 32;  // .. 34: three calls per eight sites
 
 constexpr double TWO_PI = 3.141592653589793 * 2.0;  // src/types.rs:6
 constexpr double PI = 3.141592653589793;
