@@ -552,8 +552,8 @@ int slsf_measure_correlations(slsf_batch* b, int32_t replica, int32_t rmax, int3
   double* d_z = nullptr;
   HIP_TRY(hipMallocAsync((void**)&d_xy, sizeof(double) * (size_t)std::max<int64_t>(1, rxy * rows * 3), b->stream));
   HIP_TRY(hipMallocAsync((void**)&d_z, sizeof(double) * (size_t)std::max<int64_t>(1, rz * rows), b->stream));
-  if (is_fp64(b)) launch_corr<double>(b->g, replica, rxy, rz, d_xy, d_z, b->stream);
-  else launch_corr<float>(b->g, replica, rxy, rz, d_xy, d_z, b->stream);
+  if (is_fp64(b)) launch_corr<double>(b->g, replica, rows, rxy, rz, d_xy, d_z, b->stream);
+  else launch_corr<float>(b->g, replica, rows, rxy, rz, d_xy, d_z, b->stream);
   HIP_TRY(hipGetLastError());
   std::vector<double> hxy((size_t)rxy * rows * 3), hz((size_t)rz * rows);
   if (!hxy.empty())
@@ -1059,6 +1059,7 @@ struct DriverSession {
   double* rows_xy = nullptr;  // [rxy][ly * lz][3]
   double* rows_z = nullptr;   // [rz][ly * lz]
   double* sums = nullptr;
+  double* planes = nullptr;   // [2 rxy + rz][lz] per-plane sums of one measurement
   int64_t nmeas = 0, nsums = 0;
   int rxy = 0, rz = 0, wmax = 0;
 };
@@ -1079,6 +1080,8 @@ int driver_open(slsf_batch* b, int64_t nmeas, int rxy, int rz, DriverSession** o
   if (int rc = b->alloc(&s->rows_xy, (size_t)std::max<int64_t>(1, rxy * rows * 3))) return rc;
   if (int rc = b->alloc(&s->rows_z, (size_t)std::max<int64_t>(1, rz * rows))) return rc;
   if (int rc = b->alloc(&s->sums, (size_t)std::max<int64_t>(1, s->nsums))) return rc;
+  if (int rc = b->alloc(&s->planes, (size_t)std::max<int64_t>(1, (2 * rxy + rz) * (int64_t)r.lz)))
+    return rc;
   HIP_TRY(hipMemsetAsync(s->sums, 0, sizeof(double) * (size_t)std::max<int64_t>(1, s->nsums),
                          b->stream));
   launch_clear_counters(b->g, b->stream);
@@ -1107,9 +1110,11 @@ int driver_measure(DriverSession* s, int64_t index) {
 int driver_correlations(DriverSession* s) {
   if (s->rxy + s->rz == 0) return SLSF_OK;
   slsf_batch* b = s->b;
-  if (is_fp64(b)) launch_corr<double>(b->g, 0, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
-  else launch_corr<float>(b->g, 0, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
-  launch_fold_correlations(b->g, 0, s->rows_xy, s->rows_z, s->rxy, s->rz, s->sums, b->stream);
+  const int64_t rows = (int64_t)b->reps[0].ly * b->reps[0].lz;
+  if (is_fp64(b)) launch_corr<double>(b->g, 0, rows, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
+  else launch_corr<float>(b->g, 0, rows, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
+  launch_fold_correlations(b->g, 0, s->rows_xy, s->rows_z, s->rxy, s->rz, s->planes, s->sums,
+                           b->stream);
   HIP_TRY(hipGetLastError());
   return SLSF_OK;
 }

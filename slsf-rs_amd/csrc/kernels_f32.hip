@@ -107,9 +107,10 @@ void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_
 }
 
 // ---------------------------------------------------------------------------
-// device folds of the plain driver: the same sequential FP64 sums the host
-// folds of slsf_measure_observables / slsf_measure_correlations do, so a run
-// that never reads back until its end gets identical numbers
+// device folds of the plain driver: the FP64 sums of the host folds of
+// slsf_measure_observables / slsf_measure_correlations in the same order
+// (the z-correlation sums plane by plane), so a run that never reads back
+// until its end gets the same numbers
 // ---------------------------------------------------------------------------
 __global__ void k_fold_observables(const RepDev* __restrict__ rep, const double* __restrict__ part,
                                    double* __restrict__ out) {
@@ -125,43 +126,65 @@ __global__ void k_fold_observables(const RepDev* __restrict__ rep, const double*
 }
 
 // sums[] = {corr_x[rxy], corr_y[rxy], corr_xy[rxy], corr_xy_by_z[lz * rxy], corr_z[rz]}
-// += this measurement's values; thread r - 1 < rxy takes shift r in x/y,
-// thread rxy + r - 1 shift r in z (src/observables.rs:189-252 order)
-__global__ void k_fold_correlations(const RepDev* __restrict__ rep,
-                                    const double* __restrict__ hxy, const double* __restrict__ hz,
-                                    int rxy, int rz, double* __restrict__ sums) {
+// += this measurement's values, in two steps of one block: per (shift, plane)
+// the sums over the plane's rows (the x/y sums exactly as the host fold of
+// slsf_measure_correlations nests them, src/observables.rs:189-252), then per
+// shift the sums over the planes in order
+__global__ void __launch_bounds__(256)
+k_fold_correlations(const RepDev* __restrict__ rep, const double* __restrict__ hxy,
+                    const double* __restrict__ hz, int rxy, int rz, double* __restrict__ planes,
+                    double* __restrict__ sums) {
   const RepDev d = *rep;
-  const int task = blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t rows = (int64_t)d.ly * d.lz;
-  const double vol = (double)d.volume, area = (double)d.lx * d.ly;
-  double* cx = sums;
-  double* cy = cx + rxy;
-  double* cxy = cy + rxy;
-  double* cbz = cxy + rxy;
-  double* cz = cbz + (int64_t)d.lz * rxy;
-  if (task < rxy) {
-    const int rr = task + 1;
-    double sx = 0, sy = 0;
-    for (int z = 0; z < d.lz; ++z) {
+  const int lz = d.lz, ly = d.ly;
+  const int64_t rows = (int64_t)ly * lz;
+  const double vol = (double)d.volume, area = (double)d.lx * ly;
+  double* px = planes;                     // [rxy][lz]
+  double* py = px + (int64_t)rxy * lz;     // [rxy][lz]
+  double* pz = py + (int64_t)rxy * lz;     // [rz][lz]
+  const int tasks = (rxy + rz) * lz;
+  for (int t = threadIdx.x; t < tasks; t += blockDim.x) {
+    const int rr = t / lz, z = t - rr * lz;
+    if (rr < rxy) {
       double lsx = 0, lsy = 0;
-      for (int y = 0; y < d.ly; ++y) {
-        const double* o = hxy + ((int64_t)(rr - 1) * rows + (int64_t)z * d.ly + y) * 3;
+      for (int y = 0; y < ly; ++y) {
+        const double* o = hxy + ((int64_t)rr * rows + (int64_t)z * ly + y) * 3;
         lsx += o[0];
         lsx += o[1];
         lsy += o[2];
       }
-      sx += lsx;
-      sy += lsy;
-      cbz[(int64_t)z * rxy + (rr - 1)] += (lsx + lsy) / (2.0 * area);
+      px[(int64_t)rr * lz + z] = lsx;
+      py[(int64_t)rr * lz + z] = lsy;
+    } else {
+      const int rzi = rr - rxy;
+      double s = 0;
+      for (int y = 0; y < ly; ++y) s += hz[(int64_t)rzi * rows + (int64_t)z * ly + y];
+      pz[(int64_t)rzi * lz + z] = s;
     }
-    cx[rr - 1] += sx / vol;
-    cy[rr - 1] += sy / vol;
-    cxy[rr - 1] += (sx / vol + sy / vol) / 2.0;
-  } else if (task < rxy + rz) {
-    const int rr = task - rxy + 1;
-    double sz = 0;
-    for (int64_t k = 0; k < rows; ++k) sz += hz[(int64_t)(rr - 1) * rows + k];
-    cz[rr - 1] += sz / vol;
+  }
+  __syncthreads();
+  double* cx = sums;
+  double* cy = cx + rxy;
+  double* cxy = cy + rxy;
+  double* cbz = cxy + rxy;
+  double* cz = cbz + (int64_t)lz * rxy;
+  for (int t = threadIdx.x; t < rxy + rz; t += blockDim.x) {
+    if (t < rxy) {
+      double sx = 0, sy = 0;
+      for (int z = 0; z < lz; ++z) {
+        const double lsx = px[(int64_t)t * lz + z], lsy = py[(int64_t)t * lz + z];
+        sx += lsx;
+        sy += lsy;
+        cbz[(int64_t)z * rxy + t] += (lsx + lsy) / (2.0 * area);
+      }
+      cx[t] += sx / vol;
+      cy[t] += sy / vol;
+      cxy[t] += (sx / vol + sy / vol) / 2.0;
+    } else {
+      const int rzi = t - rxy;
+      double sz = 0;
+      for (int z = 0; z < lz; ++z) sz += pz[(int64_t)rzi * lz + z];
+      cz[rzi] += sz / vol;
+    }
   }
 }
 
@@ -171,10 +194,10 @@ void launch_fold_observables(const GlobalState& g, int replica, double* out, hip
 }
 
 void launch_fold_correlations(const GlobalState& g, int replica, const double* hxy,
-                              const double* hz, int rxy, int rz, double* sums, hipStream_t s) {
+                              const double* hz, int rxy, int rz, double* planes, double* sums,
+                              hipStream_t s) {
   if (rxy + rz > 0)
-    k_fold_correlations<<<(rxy + rz + 63) / 64, 64, 0, s>>>(g.reps + replica, hxy, hz, rxy, rz,
-                                                            sums);
+    k_fold_correlations<<<1, 256, 0, s>>>(g.reps + replica, hxy, hz, rxy, rz, planes, sums);
 }
 
 }  // namespace slsf

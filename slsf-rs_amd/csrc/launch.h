@@ -51,7 +51,8 @@ void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s
 // measure_from) compute their partials; 0: every replica (a direct measurement)
 template <typename R> void launch_observables(const GlobalState& g, hipStream_t s,
                                               int windowed = 0);
-template <typename R> void launch_corr(const GlobalState& g, int replica, int rxy, int rz,
+// rows = ly * lz of that replica (sizes the grid)
+template <typename R> void launch_corr(const GlobalState& g, int replica, int64_t rows, int rxy, int rz,
                                        double* out_xy, double* out_z, hipStream_t s);
 // zeroes the accepted-move and cluster-site tallies
 void launch_clear_counters(const GlobalState& g, hipStream_t s);
@@ -62,8 +63,10 @@ void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_
 // {energy, m^2, cos_x, cos_y, cos_z, sin_x, sin_y, sin_z}) and correlation
 // rows (after launch_corr: added into sums) in the host folds' order
 void launch_fold_observables(const GlobalState& g, int replica, double* out, hipStream_t s);
+// planes: (2 rxy + rz) * lz doubles of scratch for the per-plane sums
 void launch_fold_correlations(const GlobalState& g, int replica, const double* hxy,
-                              const double* hz, int rxy, int rz, double* sums, hipStream_t s);
+                              const double* hz, int rxy, int rz, double* planes, double* sums,
+                              hipStream_t s);
 
 // device bins of a measured run; accum == nullptr runs without measurement
 struct BinTarget {
