@@ -238,7 +238,9 @@ typedef struct {
 } slsf_run_stats;
 
 /* Runs thermalization + sweeps for every replica, accumulating FP64 bin means
- * of every series on the device.  May be called repeatedly; bins append. */
+ * of every series on the device.  May be called repeatedly; bins append, and a
+ * bin left open by one run (fewer than binsize measurements) is continued by
+ * the next. */
 int slsf_run(slsf_batch* batch, const slsf_run_options* opt, slsf_run_stats* stats);
 
 /* Autotuner of the FP32 batched kernel.  For every replica size of the batch

@@ -663,8 +663,6 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
   if (opt->thermalization < 0 || opt->sweeps < 0)
     return fail(SLSF_E_INVALID, "sweep counts must be nonnegative");
   if (opt->sweeps > 0 && opt->binsize <= 0) return fail(SLSF_E_INVALID, "binsize must be positive");
-  if (opt->sweeps > 0 && opt->sweeps < opt->binsize)
-    return fail(SLSF_E_INVALID, "sweeps must be at least binsize");
   if (opt->stream != SLSF_STREAM_PRODUCTION && opt->stream != SLSF_STREAM_REPLAY)
     return fail(SLSF_E_INVALID, "stream must be SLSF_STREAM_PRODUCTION or SLSF_STREAM_REPLAY");
   if (int rc = ensure_bins(b, opt)) return rc;

@@ -27,9 +27,12 @@
 
 An estimate is written as ``{"mean", "error", "internal_bin_len"}``, the
 fields of carlo-mc's ``Estimate`` the reference itself reads
-(src/results.rs:74-83; src/result_tools.rs:200-207).  Series the batched kernels
-do not measure -- the correlation functions and the ``_ll_*`` timers -- are
-absent from ``observables``.
+(src/results.rs:74-83; src/result_tools.rs:200-207).  Tasks with correlations
+enabled also get the reference's ``CorrX_r*``, ``CorrY_r*``, ``CorrXY_r*``,
+``CorrXY_z{z}_r*`` and ``CorrZ_r*`` series (src/model.rs:410-435): measured on
+the device on every ``correlation_interval``-th measurement (the first
+included), the batch stepping to each of those sweeps, and binned like every
+series.  Only the ``_ll_*`` timers are absent.
 """
 from __future__ import annotations
 
@@ -88,7 +91,81 @@ def _rank_batch(job: Job, rank: int, world_size: int, precision: int):
     return idx, tasks, Batch(reps, precision)
 
 
-def _task_results(b: Batch, idx, tasks, measured: int, binsize: int) -> list:
+def _corr_spec(t: dict):
+    """Shifts and interval of a task's correlation measurement (src/model.rs:360-365)."""
+    rxy = min(int(t["correlation_rmax_xy"]), int(t["l_x"]) // 2, int(t["l_y"]) // 2)
+    rz = min(int(t["correlation_rmax_z"]), int(t["l_z"]) // 2)
+    return rxy, rz, max(1, int(t["correlation_interval"]))
+
+
+def _corr_names(rxy: int, rz: int, lz: int) -> list:
+    """Series names in the order src/model.rs:419-435 measures them."""
+    r = range(1, rxy + 1)
+    return ([f"CorrX_r{k}" for k in r] + [f"CorrY_r{k}" for k in r] +
+            [f"CorrXY_r{k}" for k in r] +
+            [f"CorrXY_z{z}_r{k}" for z in range(lz) for k in r] +
+            [f"CorrZ_r{k}" for k in range(1, rz + 1)])
+
+
+class _Correlations:
+    """Correlation samples of the tasks that ask for them, taken on the
+    measured sweeps with index m % interval == 0 (m counts from 0)."""
+
+    def __init__(self, tasks):
+        self.spec = [_corr_spec(t) for t in tasks]
+        self.on = [rxy > 0 or rz > 0 for rxy, rz, _ in self.spec]
+        ivals = {c for (_, _, c), on in zip(self.spec, self.on) if on}
+        if len(ivals) > 1:
+            raise SlsfError(1, "tasks of one batch must share correlation_interval")
+        self.interval = ivals.pop() if ivals else 0
+        self.samples = [[] for _ in tasks]
+
+    def sample(self, b: Batch):
+        for k, ((rxy, rz, _), on) in enumerate(zip(self.spec, self.on)):
+            if not on:
+                continue
+            c = b.correlations(k, -1, rxy, rz)
+            self.samples[k].append(np.concatenate(
+                [c["corr_x"], c["corr_y"], c["corr_xy"], c["corr_xy_by_z"].ravel(), c["corr_z"]]))
+
+    def run(self, b: Batch, start: int, n: int, binsize: int):
+        """Measured sweeps start .. start+n-1 (indices of measurements)."""
+        if not self.interval:
+            b.run(0, n, binsize)
+            return
+        done = 0
+        while done < n:
+            m = start + done
+            nxt = m + (-m) % self.interval  # next measurement taking correlations
+            if nxt >= start + n:
+                b.run(0, start + n - m, binsize)
+                break
+            b.run(0, nxt - m + 1, binsize)
+            self.sample(b)
+            done = nxt - start + 1
+
+    def state(self) -> dict:
+        return {f"corr{k}": np.array(v) for k, v in enumerate(self.samples) if v}
+
+    def restore(self, arrays) -> None:
+        for k in range(len(self.samples)):
+            key = f"corr{k}"
+            self.samples[k] = list(arrays[key]) if key in arrays else []
+
+    def series(self, k: int, lz: int, binsize: int) -> dict:
+        """Bin means of task k's correlation series (complete bins only)."""
+        if not self.on[k]:
+            return {}
+        rxy, rz, _ = self.spec[k]
+        names = _corr_names(rxy, rz, lz)
+        v = np.array(self.samples[k]).reshape(-1, len(names))
+        nb = v.shape[0] // binsize
+        bins = v[:nb * binsize].reshape(nb, binsize, len(names)).mean(axis=1)
+        return {name: bins[:, j] for j, name in enumerate(names)}
+
+
+def _task_results(b: Batch, idx, tasks, measured: int, binsize: int,
+                  corr: "_Correlations | None" = None) -> list:
     """ThetaTaskResult entries of a batch that has measured `measured` sweeps."""
     out = []
     acceptance = b.acceptance()
@@ -103,14 +180,19 @@ def _task_results(b: Batch, idx, tasks, measured: int, binsize: int) -> list:
                 mean, err = ev[name]
                 obs[name] = {"mean": float(mean), "error": float(err),
                              "internal_bin_len": binsize}
+        mbins = {name: np.array(bins[s], dtype=np.float64)
+                 for s, name in enumerate(SERIES) if bins.shape[1] > 0}
+        if corr is not None:
+            for name, v in corr.series(k, int(t["l_z"]), binsize).items():
+                obs[name] = _binned(v, binsize)
+                if len(v):
+                    mbins[name] = v
         entry = {"task": _task_json(t)}
         if i != 0:
             entry["task_index"] = int(i)  # serialised only when nonzero
         entry.update({"observables": dict(sorted(obs.items())),
                       "acceptance": float(acceptance[k]), "measurements": measured,
-                      "measurement_bins": {name: np.array(bins[s], dtype=np.float64)
-                                           for s, name in enumerate(SERIES)
-                                           if bins.shape[1] > 0}})
+                      "measurement_bins": mbins})
         out.append(entry)
     return out
 
@@ -123,13 +205,16 @@ def run_job(job: Job, rank: int = 0, world_size: int = 1, precision: int = FP32,
     if b is None:
         return result
     therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
+    corr = _Correlations(tasks)
     try:
         b.init_angles()
         if therm:
             b.run(therm, 0, 1, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
-        if sweeps:
+        if sweeps and corr.interval:
+            corr.run(b, 0, sweeps, binsize)
+        elif sweeps:
             b.run(0, sweeps, binsize, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
-        result["tasks"] = _task_results(b, idx, tasks, sweeps, binsize)
+        result["tasks"] = _task_results(b, idx, tasks, sweeps, binsize, corr)
     finally:
         b.close()
     return result
@@ -140,10 +225,14 @@ def checkpoint_path(job_name: str, rank: int, output_dir: str | None = None) -> 
     return os.path.join(measurement_dir(job_name, output_dir), f"rank{rank}.checkpoint")
 
 
-def _save_image(b: Batch, path: str) -> None:
+def _save_image(b: Batch, path: str, corr: "_Correlations | None" = None) -> None:
     parent = os.path.dirname(path)
     if parent:
         os.makedirs(parent, exist_ok=True)
+    if corr is not None and corr.interval:  # host-side correlation samples beside the image
+        tmp = path + ".corr.tmp.npz"
+        np.savez(tmp, **corr.state())
+        os.replace(tmp, path + ".corr.npz")
     tmp = path + ".tmp"
     b.checkpoint().tofile(tmp)
     os.replace(tmp, path)
@@ -179,9 +268,13 @@ def run_job_resumable(job: Job, path: str, rank: int = 0, world_size: int = 1,
     if b is None:
         return result, summary
     therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
+    corr = _Correlations(tasks)
     try:
         if os.path.exists(path):
             b.restore(np.fromfile(path, dtype=np.uint8))
+            if corr.interval and os.path.exists(path + ".corr.npz"):
+                with np.load(path + ".corr.npz") as f:
+                    corr.restore(f)
         else:
             b.init_angles()
         done = int(b.counters()[0][0])
@@ -195,14 +288,14 @@ def run_job_resumable(job: Job, path: str, rank: int = 0, world_size: int = 1,
                 b.run(n, 0, 1)
             else:
                 n = min(chunk_sweeps, therm + sweeps - done)
-                b.run(0, n, binsize)
+                corr.run(b, done - therm, n, binsize)
             done += n
             if time.monotonic() - last >= checkpoint_time:
-                _save_image(b, path)
+                _save_image(b, path, corr)
                 last = time.monotonic()
-        _save_image(b, path)
+        _save_image(b, path, corr)
         summary["checkpoint_paths"] = [path]
-        result["tasks"] = _task_results(b, idx, tasks, max(0, done - therm), binsize)
+        result["tasks"] = _task_results(b, idx, tasks, max(0, done - therm), binsize, corr)
     finally:
         b.close()
     summary["elapsed_seconds"] = time.monotonic() - started

@@ -69,7 +69,10 @@ def test_run_job_measurement_files(tmp_path):
     assert len(paths) == 2
     for tr, p in zip(res["tasks"], paths):
         assert p.endswith(os.path.join(f"task{tr['task_index'] + 1:04d}", "run0001.meas.h5"))
-        assert sorted(tr["measurement_bins"]) == sorted(slsf.SERIES)
+        # the 13 series and, correlations being on by default (interval 1), the Corr series
+        t = tr["task"]
+        corr = R._corr_names(t["l_x"] // 2, t["l_z"] // 2, t["l_z"])
+        assert sorted(tr["measurement_bins"]) == sorted(list(slsf.SERIES) + corr)
         assert all(len(v) == 6 for v in tr["measurement_bins"].values())
     h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
     if os.path.exists(h5dump):
@@ -77,3 +80,39 @@ def test_run_job_measurement_files(tmp_path):
         subprocess.run([h5dump, "-d", "/observables/Energy/samples", "-b", "LE", "-o", str(out),
                         paths[1]], check=True, capture_output=True)
         assert out.read_bytes() == res["tasks"][1]["measurement_bins"]["Energy"].tobytes()
+
+
+JOB_CORR = ('name = "corr_unit"\n[model]\nL=[8]\nT=[1.3]\nsamples=1\nbase_seed=9\n'
+            '[run]\nsweeps=20\nthermalization=10\nbinsize=2\n'
+            '[measure]\ncorr_rmax_xy=[3]\ncorr_rmax_z=[2]\ncorrelation_interval=3\n')
+
+
+def test_correlation_series_follow_the_reference_cadence():
+    """CorrX/CorrY/CorrXY/CorrXY_z*/CorrZ series (src/model.rs:410-435): taken
+    on measurements 0, 3, 6, ... of the task's own chain and binned like every
+    series.  An independent stepping of the same replica, one measured sweep
+    at a time, gives exactly the same bins."""
+    job = slsf.Job(JOB_CORR)
+    t0 = job.task(0)
+    assert (t0["correlation_rmax_xy"], t0["correlation_rmax_z"], t0["correlation_interval"]) \
+        == (3, 2, 3)
+    res = R.run_job(job)
+    tr = res["tasks"][0]
+    corr = sorted(n for n in tr["observables"] if n.startswith("Corr"))
+    assert corr == sorted(R._corr_names(3, 2, 8))
+    assert len(corr) == 3 * 3 + 8 * 3 + 2
+    b = slsf.Batch([job.replica(0)], slsf.FP32)
+    b.init_angles()
+    b.run(10, 0, 1)
+    samples = []
+    for m in range(20):
+        b.run(0, 1, 2)
+        if m % 3 == 0:
+            c = b.correlations(0, -1, 3, 2)
+            samples.append((c["corr_x"][1], c["corr_xy_by_z"][5, 2], c["corr_z"][0]))
+    s = np.array(samples[:6]).reshape(3, 2, 3).mean(axis=1)  # 7 samples: 3 full bins of 2
+    for j, name in enumerate(("CorrX_r2", "CorrXY_z5_r3", "CorrZ_r1")):
+        assert np.array_equal(tr["measurement_bins"][name], s[:, j]), name
+        assert tr["observables"][name]["mean"] == float(np.mean(s[:, j])), name
+        assert tr["observables"][name]["internal_bin_len"] == 2
+    assert np.array_equal(b.all_bins(0)[0], tr["measurement_bins"]["Energy"])
