@@ -147,10 +147,20 @@ class _Correlations:
     def state(self) -> dict:
         return {f"corr{k}": np.array(v) for k, v in enumerate(self.samples) if v}
 
-    def restore(self, arrays) -> None:
+    def restore(self, arrays, measured: int) -> None:
+        """Samples saved beside a checkpoint of a batch that has measured
+        `measured` sweeps: one per measurement m < measured with m % interval
+        == 0, for every task that takes them."""
+        want = -(-measured // self.interval) if self.interval else 0
         for k in range(len(self.samples)):
             key = f"corr{k}"
-            self.samples[k] = list(arrays[key]) if key in arrays else []
+            # the samples are saved before the image, so they may run ahead of it
+            self.samples[k] = (list(arrays[key])[:want]
+                               if arrays is not None and key in arrays else [])
+            if self.on[k] and len(self.samples[k]) != want:
+                raise SlsfError(1, f"the checkpoint's correlation samples of task {k} "
+                                   f"({len(self.samples[k])}) do not match its {measured} "
+                                   "measured sweeps")
 
     def series(self, k: int, lz: int, binsize: int) -> dict:
         """Bin means of task k's correlation series (complete bins only)."""
@@ -272,12 +282,16 @@ def run_job_resumable(job: Job, path: str, rank: int = 0, world_size: int = 1,
     try:
         if os.path.exists(path):
             b.restore(np.fromfile(path, dtype=np.uint8))
-            if corr.interval and os.path.exists(path + ".corr.npz"):
-                with np.load(path + ".corr.npz") as f:
-                    corr.restore(f)
         else:
             b.init_angles()
         done = int(b.counters()[0][0])
+        if corr.interval:
+            side = path + ".corr.npz"
+            if os.path.exists(side):
+                with np.load(side) as f:
+                    corr.restore(f, max(0, done - therm))
+            else:
+                corr.restore(None, max(0, done - therm))
         last = time.monotonic()
         while done < therm + sweeps:
             if time.monotonic() - started >= run_time:
