@@ -109,7 +109,28 @@ def cpu_baseline(budget_s=12.0):
                       f"sweeps each ({therm} thermalization, then measured every sweep), FP64, "
                       "reference draw order",
             "per_core": float(np.mean([o[0] for o in out])),
+            "cpu_model": _cpu_model(), "smt_active": _smt_active(),
+            "host_cpus": os.cpu_count(),
             "bins": {i: o[1] for i, o in enumerate(out)}}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _smt_active():
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            return f.read().strip() == "1"
+    except OSError:
+        return None
 
 
 def _cpu_worker(arg):
