@@ -1117,6 +1117,58 @@ int driver_correlations(DriverSession* s) {
   return SLSF_OK;
 }
 
+// sweeps per launch of the one-workgroup replay: about 10-20 ms of work at
+// L = 16, so a long run is still cut into launches the host can interleave
+constexpr int64_t REPLAY_CHUNK = 1024;
+
+int driver_run(DriverSession* s, const DriverSchedule& plan, int64_t* ncorr) {
+  if (plan.thermal < 0 || plan.measure_sweeps < 0 || plan.measure_interval <= 0 ||
+      plan.correlation_interval <= 0)
+    return fail(SLSF_E_INVALID, "invalid driver schedule");
+  slsf_batch* b = s->b;
+  const int64_t nmeas = plan.measure_sweeps / plan.measure_interval;
+  if (nmeas > s->nmeas) return fail(SLSF_E_INVALID, "more measurements than the session holds");
+  const bool corr = plan.corr && s->rxy + s->rz > 0;
+  const auto& r = b->reps_host[0];
+  if (is_fp64(b) && b->reps.size() == 1 && replay_resident_fits(r.lx, r.ly, r.lz) &&
+      replay_resident_fits_corr(r.lz, s->rxy, s->rz)) {
+    ReplayRun run;
+    run.measure_interval = plan.measure_interval;
+    run.correlation_interval = plan.correlation_interval;
+    run.rxy = s->rxy;
+    run.rz = s->rz;
+    run.corr = corr ? 1 : 0;
+    run.series = s->series;
+    run.rows_xy = s->rows_xy;
+    run.rows_z = s->rows_z;
+    run.planes = s->planes;
+    run.sums = s->sums;
+    const int64_t total = plan.thermal + plan.measure_sweeps;
+    for (int64_t done = 0; done < total; done += REPLAY_CHUNK) {
+      run.nsweeps = std::min(REPLAY_CHUNK, total - done);
+      run.step0 = done - plan.thermal + 1;
+      launch_replay_resident(b->g, 0, run, b->stream);
+      HIP_TRY(hipGetLastError());
+    }
+    *ncorr = corr ? (nmeas + plan.correlation_interval - 1) / plan.correlation_interval : 0;
+    return SLSF_OK;
+  }
+  if (int rc = driver_sweeps(s, (int)plan.thermal)) return rc;
+  int64_t idx = 0, nc = 0;
+  for (int64_t step = 1; step <= plan.measure_sweeps; ++step) {
+    if (int rc = driver_sweeps(s, 1)) return rc;
+    if (step % plan.measure_interval != 0) continue;
+    if (int rc = driver_measure(s, idx)) return rc;
+    if (corr && idx % plan.correlation_interval == 0) {
+      if (int rc = driver_correlations(s)) return rc;
+      ++nc;
+    }
+    ++idx;
+  }
+  *ncorr = nc;
+  return SLSF_OK;
+}
+
 int driver_close(DriverSession* s, int64_t nsweeps, double* acceptance_sum, double* series,
                  double* sums) {
   slsf_batch* b = s->b;

@@ -104,22 +104,15 @@ extern "C" int slsf_run_theta_simulation(int32_t l_x, int32_t l_y, int32_t l_z, 
     return rc;
   // The whole run is queued on the batch's stream: measurements go to device
   // buffers (the host folds' order, on the device) and are read back once.
-  const bool corr = rxy > 0 || rz > 0;
   if (int rc = driver_open(g.b, nmeas, rxy, rz, &g.s)) return rc;
-  if (o->thermal_sweeps > 0)
-    if (int rc = driver_sweeps(g.s, o->thermal_sweeps)) return rc;
-  int ncorr = 0;
-  int64_t idx = 0;
-  for (int step = 1; step <= o->measure_sweeps; ++step) {
-    if (int rc = driver_sweeps(g.s, 1)) return rc;
-    if (step % o->measure_interval != 0) continue;
-    if (int rc = driver_measure(g.s, idx)) return rc;
-    ++idx;
-    if (corr && (idx - 1) % o->correlation_interval == 0) {
-      if (int rc = driver_correlations(g.s)) return rc;
-      ++ncorr;
-    }
-  }
+  DriverSchedule plan;
+  plan.thermal = o->thermal_sweeps > 0 ? o->thermal_sweeps : 0;
+  plan.measure_sweeps = o->measure_sweeps;
+  plan.measure_interval = o->measure_interval;
+  plan.correlation_interval = o->correlation_interval;
+  plan.corr = rxy > 0 || rz > 0;
+  int64_t ncorr = 0;
+  if (int rc = driver_run(g.s, plan, &ncorr)) return rc;
   std::vector<double> series((size_t)nmeas * 8), sums((size_t)driver_sums_size(g.s));
   double acc_sum = 0.0;
   if (int rc = driver_close(g.s, (int64_t)o->thermal_sweeps + o->measure_sweeps, &acc_sum,
@@ -176,6 +169,6 @@ extern "C" int slsf_run_theta_simulation(int32_t l_x, int32_t l_y, int32_t l_z, 
   const int count = o->thermal_sweeps + o->measure_sweeps;
   res->acceptance = count == 0 ? 0.0 : acc_sum / (double)count;
   res->num_measurements = nmeas;
-  res->num_correlation_measurements = ncorr;
+  res->num_correlation_measurements = (int32_t)ncorr;
   return SLSF_OK;
 }

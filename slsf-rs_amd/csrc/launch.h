@@ -59,6 +59,41 @@ void launch_clear_counters(const GlobalState& g, hipStream_t s);
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
                        int32_t bin_cap, hipStream_t s);
 
+// The FP64 replay of one small replica inside one workgroup
+// (k_replay_resident): a chunk of `nsweeps` sweeps of the plain driver, with
+// its measurements and correlation measurements.  Sweep j of the chunk is
+// step step0 + j of the measurement phase (steps <= 0 thermalise); a step >= 1
+// with step % measure_interval == 0 is measurement step / measure_interval - 1,
+// written to series[8 m ..] as launch_fold_observables writes it, and with
+// corr set, measurement m with m % correlation_interval == 0 adds its
+// correlations to sums as launch_fold_correlations does (rows_xy, rows_z and
+// planes are that fold's scratch).  Tallies and counters advance as on the
+// HBM path.
+struct ReplayRun {
+  int64_t nsweeps = 0, step0 = 0;
+  int32_t measure_interval = 1, correlation_interval = 1;
+  int32_t rxy = 0, rz = 0, corr = 0, pad = 0;
+  double* series = nullptr;
+  double* rows_xy = nullptr;
+  double* rows_z = nullptr;
+  double* planes = nullptr;
+  double* sums = nullptr;
+};
+constexpr int RR_THREADS = 512;
+constexpr int RR_MAX_SITES = 4096;  // angles + (cos, sin) pairs: 96 KiB of LDS
+constexpr int RR_MAX_ROWS = 256;    // x-rows of the cluster's bitmaps
+constexpr int RR_MAX_CORR_SUMS = 1136;  // 3 rxy + Lz rxy + rz with rxy <= Lx/2, Lz Lx <= 2048
+inline bool replay_resident_fits(int lx, int ly, int lz) {
+  const int64_t v = (int64_t)lx * ly * lz, rows = (int64_t)ly * lz;
+  return lx >= 2 && lx <= 32 && lx % 2 == 0 && ly % 2 == 0 && lz % 2 == 0 &&
+         v <= RR_MAX_SITES && rows <= RR_MAX_ROWS && (int64_t)lx * lz <= RR_THREADS;
+}
+inline bool replay_resident_fits_corr(int lz, int rxy, int rz) {
+  return 3 * rxy + lz * rxy + rz <= RR_MAX_CORR_SUMS;
+}
+void launch_replay_resident(const GlobalState& g, int replica, const ReplayRun& run,
+                            hipStream_t s);
+
 // folds of one replica's observables (after launch_observables: out[8] =
 // {energy, m^2, cos_x, cos_y, cos_z, sin_x, sin_y, sin_z}) and correlation
 // rows (after launch_corr: added into sums) in the host folds' order

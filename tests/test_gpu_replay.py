@@ -228,6 +228,34 @@ def test_driver_c1_matches_oracle_replay():
     assert np.allclose(g["corr_xy_by_z"].ravel(), bufs[3][:L * 8], atol=1e-12)
 
 
+# Shapes of the one-workgroup replay (Lx a divisor of 64 or not, Lx = 32, up to
+# four x-rows per lane of the fill wave) and one too large for it (HBM kernels).
+@pytest.mark.parametrize("dims,mi,ci", [((12, 12, 12), 1, 1), ((32, 4, 8), 2, 3),
+                                        ((6, 8, 10), 3, 2), ((4, 4, 4), 1, 1),
+                                        ((18, 18, 18), 2, 2)])
+def test_driver_shapes_match_oracle_replay(dims, mi, ci):
+    lx, ly, lz = dims
+    T, seed = 1.1, 7
+    jxy = orc.uniform_layers(lz, 0.6, 1.8, 3)
+    jz = np.linspace(0.5, 1.5, lz)
+    opt = dict(thermal_sweeps=20, measure_sweeps=60, measure_interval=mi, correlation_rmax=4,
+               correlation_interval=ci, wolff_steps=2)
+    g = slsf.run_theta_simulation(lx, ly, lz, 1.0, 1.0, 0.0, T, seed=seed, j_xy_array=jxy,
+                                  j_z_array=jz, **opt)
+    o, bufs = _oracle_driver(dims, T, jxy, jz, seed, **opt)
+    for k in ("energy", "rho_sx", "rho_sy", "rho_sz", "std_rho_sx", "std_rho_sy", "std_rho_sz",
+              "magnetization_squared", "chi", "acceptance"):
+        ov = getattr(o, k)
+        assert abs(g[k] - ov) <= 1e-10 * max(1.0, abs(ov)), (dims, k)
+    assert g["num_measurements"] == o.num_measurements == 60 // mi
+    assert g["num_correlation_measurements"] == o.num_correlation_measurements
+    rxy, rz = len(g["corr_x"]), len(g["corr_z"])
+    assert np.allclose(g["corr_x"], bufs[0][:rxy], atol=1e-12)
+    assert np.allclose(g["corr_y"], bufs[1][:rxy], atol=1e-12)
+    assert np.allclose(g["corr_z"], bufs[4][:rz], atol=1e-12)
+    assert np.allclose(g["corr_xy_by_z"].ravel(), bufs[3][:lz * rxy], atol=1e-12)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_driver_reference_assertions(precision):  # src/tests.rs:241-298,333-353
     r = slsf.run_theta_simulation(4, 4, 4, 1.0, 1.0, 0.0, 0.5, thermal_sweeps=5,
