@@ -228,10 +228,13 @@ def test_driver_c1_matches_oracle_replay():
     assert np.allclose(g["corr_xy_by_z"].ravel(), bufs[3][:L * 8], atol=1e-12)
 
 
-# Shapes of the one-workgroup replay (Lx a divisor of 64 or not, Lx = 32, up to
-# four x-rows per lane of the fill wave) and one too large for it (HBM kernels).
+# Shapes of the one-workgroup replay (Lx a divisor of 64 or not, Lx = 32, plane
+# sums over x from lane groups of 2, 4, 8 and 16 or from the scratch fold) and
+# one too large for it (HBM kernels).
 @pytest.mark.parametrize("dims,mi,ci", [((12, 12, 12), 1, 1), ((32, 4, 8), 2, 3),
                                         ((6, 8, 10), 3, 2), ((4, 4, 4), 1, 1),
+                                        ((8, 8, 8), 1, 1), ((16, 8, 12), 1, 2),
+                                        ((2, 4, 32), 2, 1), ((4, 8, 16), 1, 1),
                                         ((18, 18, 18), 2, 2)])
 def test_driver_shapes_match_oracle_replay(dims, mi, ci):
     lx, ly, lz = dims
