@@ -325,6 +325,10 @@ def bench_c3(args, slsf):
 def bench_c1(args, slsf):
     L, T = 16, 1.3
     jxy = splitmix_uniform_layers(L, 1)
+    # untimed warmup: a short run of the same driver loads the kernels
+    slsf.run_theta_simulation(L, L, L, 1.2, 1.0, 0.0, T, thermal_sweeps=10, measure_sweeps=10,
+                              measure_interval=1, correlation_rmax=8, seed=2, j_xy_array=jxy,
+                              j_z_array=np.ones(L), precision=slsf.FP64_REPLAY)
     t0 = time.perf_counter()
     r = slsf.run_theta_simulation(L, L, L, 1.2, 1.0, 0.0, T, thermal_sweeps=1000,
                                   measure_sweeps=5000, measure_interval=1, correlation_rmax=8,

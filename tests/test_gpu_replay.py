@@ -235,6 +235,7 @@ def test_driver_c1_matches_oracle_replay():
                                         ((6, 8, 10), 3, 2), ((4, 4, 4), 1, 1),
                                         ((8, 8, 8), 1, 1), ((16, 8, 12), 1, 2),
                                         ((2, 4, 32), 2, 1), ((4, 8, 16), 1, 1),
+                                        ((16, 16, 8), 1, 2),
                                         ((18, 18, 18), 2, 2)])
 def test_driver_shapes_match_oracle_replay(dims, mi, ci):
     lx, ly, lz = dims
