@@ -260,6 +260,27 @@ def test_driver_shapes_match_oracle_replay(dims, mi, ci):
     assert np.allclose(g["corr_xy_by_z"].ravel(), bufs[3][:lz * rxy], atol=1e-12)
 
 
+def test_driver_chunked_run_matches_oracle_replay():
+    """A run longer than one launch of the one-workgroup replay (1024 sweeps):
+    measurement and correlation indices carry across the launch boundaries."""
+    dims, T, seed = (4, 4, 6), 0.9, 5
+    jxy = orc.uniform_layers(dims[2], 0.6, 1.8, 9)
+    jz = np.full(dims[2], 0.8)
+    opt = dict(thermal_sweeps=700, measure_sweeps=1500, measure_interval=3,
+               correlation_rmax=2, correlation_interval=2)
+    g = slsf.run_theta_simulation(*dims, 1.0, 1.0, 0.0, T, seed=seed, j_xy_array=jxy,
+                                  j_z_array=jz, **opt)
+    o, bufs = _oracle_driver(dims, T, jxy, jz, seed, **opt)
+    for k in ("energy", "rho_sx", "rho_sz", "std_rho_sx", "std_rho_sz",
+              "magnetization_squared", "acceptance"):
+        ov = getattr(o, k)
+        assert abs(g[k] - ov) <= 1e-10 * max(1.0, abs(ov)), k
+    assert g["num_measurements"] == o.num_measurements == 500
+    assert g["num_correlation_measurements"] == o.num_correlation_measurements == 250
+    assert np.allclose(g["corr_x"], bufs[0][:2], atol=1e-12)
+    assert np.allclose(g["corr_z"], bufs[4][:2], atol=1e-12)
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_driver_reference_assertions(precision):  # src/tests.rs:241-298,333-353
     r = slsf.run_theta_simulation(4, 4, 4, 1.0, 1.0, 0.0, 0.5, thermal_sweeps=5,
