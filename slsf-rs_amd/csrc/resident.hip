@@ -163,7 +163,10 @@ SLSF_D uint32_t field24(const Words (&w)[NC], int f) {
   if (off == 0) return a >> 8;
   if (off == 8) return a & 0xFFFFFFu;
   const uint32_t b = word(w[(wi + 1) >> 2], (wi + 1) & 3);
-  return __builtin_amdgcn_alignbit(a, b, (uint32_t)(off == 16 ? 24 : 16)) & 0xFFFFFFu;
+  // a field across two words is three whole bytes of them: one v_perm_b32
+  // (selector bytes 0-3 pick bytes of b, 4-7 bytes of a, 0x0C a zero byte).
+  // off 16: (a.byte1, a.byte0, b.byte3); off 24: (a.byte0, b.byte3, b.byte2)
+  return __builtin_amdgcn_perm(a, b, off == 16 ? 0x0C050403u : 0x0C040302u);
 }
 
 // hardware transcendentals: v_sin/v_cos take revolutions, v_exp is 2^x
@@ -221,14 +224,31 @@ SLSF_D void measure_site(MeasPart& m, float c, float s, float2 nxp, float2 nxm, 
 // first site); otherwise one call per site with the replay path's counters
 // (tag METRO), which lets an FP32 run consume exactly the uniforms the FP64
 // replay path uses.
+// The workgroup's LDS.  The row bitmaps and the coupling table come first, so
+// their offsets stay below 64 KiB and fold into the immediate offsets of the
+// LDS instructions; the exchange buffer is addressed from per-thread bases.
+template <int L, int CPT>
+struct ResLds {
+  using S = RShape<L, CPT>;
+  uint32_t bxm[S::ROWS], bym[S::ROWS], bzm[S::ROWS], vis[S::ROWS];
+  float4 jt[L];  // {J_xy[z], J_z[z], J_z[z-1], 0}
+  double red[S::NW * 9];
+  int flags[3];
+  float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V], then path masks
+};
+
 template <int L, int CPT, bool PAIRED>
 __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   using S = RShape<L, CPT>;
-  __shared__ float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V], then path masks
-  __shared__ uint32_t bxm[S::ROWS], bym[S::ROWS], bzm[S::ROWS], vis[S::ROWS];
-  __shared__ float4 jt[L];           // {J_xy[z], J_z[z], J_z[z-1], 0}
-  __shared__ double red[S::NW * 9];
-  __shared__ int flags[3];
+  __shared__ ResLds<L, CPT> sm;
+  float2* const xbuf = sm.xbuf;
+  uint32_t* const bxm = sm.bxm;
+  uint32_t* const bym = sm.bym;
+  uint32_t* const bzm = sm.bzm;
+  uint32_t* const vis = sm.vis;
+  float4* const jt = sm.jt;
+  double* const red = sm.red;
+  int* const flags = sm.flags;
   float* rbuf = reinterpret_cast<float*>(xbuf);
 
   const int rep = a.index[blockIdx.x];
@@ -358,8 +378,12 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       if (meas) measure_site(part, c0, s0, nxp, nxm, nyp, nym, nzp, nzm, jj);
       const float de = (c0 - c1) * hx + (s0 - s1) * hy;
       const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
-      const bool acc = (de <= 0.0f) | (ua < pacc);
-      nacc_wave += __popcll(__ballot(acc));  // scalar count per wave
+      const bool down = de <= 0.0f, up = ua < pacc;
+      const bool acc = down | up;
+      // scalar count per wave; the ballots of the two compares are their
+      // lane masks as they stand (a ballot of the combined bool would first
+      // move it through a vector register)
+      nacc_wave += __popcll(__builtin_amdgcn_ballot_w64(down) | __builtin_amdgcn_ballot_w64(up));
       th[j][p * H + k] = acc ? t1 : t0;
       // one site in flight per thread: keeps the register budget of the
       // 1024-thread L=32 workgroup (occupancy supplies the latency hiding)
@@ -485,7 +509,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         }
         const int z = zq + q;
         const int zn = (z + 1 == L) ? 0 : z + 1;
-        bool bx_ = false, by_ = false, bz_ = false;
+        unsigned long long gx = 0, gy = 0, gz = 0;  // bond ballots of the wave
         if (own) {
           const int i = x + L * (yj + L * z);
           const float ri = rbuf[i];
@@ -505,16 +529,17 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           }
           const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
           // active iff p > 0 and u < 1 - exp(-2 beta p), as u 2^24 < (1 - e) 2^24
-          // with the right side in one fma
+          // with the right side in one fma.  One compare decides both: for
+          // p <= 0, e >= 1 makes the right side <= 0 <= u 2^24.  Its ballot is
+          // the compare's lane mask (lanes outside the lattice add no bits).
           auto active = [&](float wu, float pb) {
             const float e = exp2_hw(m2b * pb);
-            return (pb > 0.0f) & (wu < __builtin_fmaf(-e, 0x1.0p24f, 0x1.0p24f));
+            return __builtin_amdgcn_ballot_w64(wu < __builtin_fmaf(-e, 0x1.0p24f, 0x1.0p24f));
           };
-          bx_ = active(wv[0], px);
-          by_ = active(wv[1], py);
-          bz_ = active(wv[2], pz);
+          gx = active(wv[0], px);
+          gy = active(wv[1], py);
+          gz = active(wv[2], pz);
         }
-        const unsigned long long gx = __ballot(bx_), gy = __ballot(by_), gz = __ballot(bz_);
         if (lane < S::RPW) {
           const int row = wave * S::RPW + lane + j * S::YS + L * z;
           const int sh = lane * S::RW;
