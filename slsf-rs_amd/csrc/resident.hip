@@ -204,16 +204,35 @@ SLSF_D void wg_sum(double (&v)[N], double* red) {
 struct MeasPart {
   float ec[3], es[3], mx, my;
 };
-SLSF_D void measure_site(MeasPart& m, float c, float s, float2 nxp, float2 nxm, float2 nyp,
-                         float2 nym, float2 nzp, float2 nzm, float4 jj) {
-  const float hzc = jj.y * nzp.x + jj.z * nzm.x, hzs = jj.y * nzp.y + jj.z * nzm.y;
-  const float dzc = jj.y * nzp.x - jj.z * nzm.x, dzs = jj.y * nzp.y - jj.z * nzm.y;
-  m.ec[0] += jj.x * (c * (nxp.x + nxm.x) + s * (nxp.y + nxm.y));
-  m.ec[1] += jj.x * (c * (nyp.x + nym.x) + s * (nyp.y + nym.y));
-  m.ec[2] += c * hzc + s * hzs;
-  m.es[0] += jj.x * (s * (nxp.x - nxm.x) - c * (nxp.y - nxm.y));
-  m.es[1] += jj.x * (s * (nyp.x - nym.x) - c * (nyp.y - nym.y));
-  m.es[2] += s * dzc - c * dzs;
+// Sums of a site's neighbour pairs shared by its local field and its
+// measurement: (cos, sin) of the two x neighbours and of the two y
+// neighbours, and the coupled z field, each with explicit fmas so that every
+// caller rounds them alike.
+struct PairSums {
+  float sxc, sxs, syc, sys, hzc, hzs;
+};
+SLSF_D PairSums pair_sums(float2 nxp, float2 nxm, float2 nyp, float2 nym, float2 nzp,
+                          float2 nzm, float4 jj) {
+  return {nxp.x + nxm.x,
+          nxp.y + nxm.y,
+          nyp.x + nym.x,
+          nyp.y + nym.y,
+          __builtin_fmaf(jj.y, nzp.x, jj.z * nzm.x),
+          __builtin_fmaf(jj.y, nzp.y, jj.z * nzm.y)};
+}
+SLSF_D void measure_site(MeasPart& m, float c, float s, const PairSums& q, float2 nxp,
+                         float2 nxm, float2 nyp, float2 nym, float2 nzp, float2 nzm,
+                         float4 jj) {
+  m.ec[0] = __builtin_fmaf(jj.x, __builtin_fmaf(c, q.sxc, s * q.sxs), m.ec[0]);
+  m.ec[1] = __builtin_fmaf(jj.x, __builtin_fmaf(c, q.syc, s * q.sys), m.ec[1]);
+  m.ec[2] = __builtin_fmaf(s, q.hzs, __builtin_fmaf(c, q.hzc, m.ec[2]));
+  m.es[0] = __builtin_fmaf(jj.x, __builtin_fmaf(s, nxp.x - nxm.x, -(c * (nxp.y - nxm.y))),
+                           m.es[0]);
+  m.es[1] = __builtin_fmaf(jj.x, __builtin_fmaf(s, nyp.x - nym.x, -(c * (nyp.y - nym.y))),
+                           m.es[1]);
+  const float dzc = __builtin_fmaf(jj.y, nzp.x, -(jj.z * nzm.x));
+  const float dzs = __builtin_fmaf(jj.y, nzp.y, -(jj.z * nzm.y));
+  m.es[2] = __builtin_fmaf(s, dzc, __builtin_fmaf(-c, dzs, m.es[2]));
   m.mx += c + nzp.x;
   m.my += s + nzp.y;
 }
@@ -352,8 +371,9 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       const float2 nzm = xbuf[bz + zo + dzm];
       const float4 jj = jz0[2 * k];
       const int row = L * (y + L * (2 * k + z0));
-      const float hx = jj.x * (nxp.x + nxm.x + nyp.x + nym.x) + jj.y * nzp.x + jj.z * nzm.x;
-      const float hy = jj.x * (nxp.y + nxm.y + nyp.y + nym.y) + jj.y * nzp.y + jj.z * nzm.y;
+      const PairSums q = pair_sums(nxp, nxm, nyp, nym, nzp, nzm, jj);
+      const float hx = __builtin_fmaf(jj.x, q.sxc + q.syc, q.hzc);
+      const float hy = __builtin_fmaf(jj.x, q.sxs + q.sys, q.hzs);
       float wp, ua;  // wp: the proposal's 24-bit integer as a float
       if (PAIRED) {
         // group of sites k0g .. k0g + 7 (fewer at the column's end)
@@ -375,7 +395,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       // 2u - 1 with u = wp 2^-24, in one exact fma
       const float t1 = fract_hw(t0 + wturn * __builtin_fmaf(wp, 0x1.0p-23f, -1.0f));
       const float c0 = cos_t(t0), s0 = sin_t(t0), c1 = cos_t(t1), s1 = sin_t(t1);
-      if (meas) measure_site(part, c0, s0, nxp, nxm, nyp, nym, nzp, nzm, jj);
+      if (meas) measure_site(part, c0, s0, q, nxp, nxm, nyp, nym, nzp, nzm, jj);
       const float de = (c0 - c1) * hx + (s0 - s1) * hy;
       const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
       const bool down = de <= 0.0f, up = ua < pacc;
@@ -450,7 +470,9 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         const int dzm = (k == 0) ? (z0 ? -HL : HL * (L - 1)) : -HL;
         const float2 nzp = xbuf[bz + zo + dzp];
         const float2 nzm = xbuf[bz + zo + dzm];
-        measure_site(part, cos_t(t0), sin_t(t0), nxp, nxm, nyp, nym, nzp, nzm, jz0[2 * k]);
+        const float4 jj = jz0[2 * k];
+        measure_site(part, cos_t(t0), sin_t(t0), pair_sums(nxp, nxm, nyp, nym, nzp, nzm, jj),
+                     nxp, nxm, nyp, nym, nzp, nzm, jj);
         __builtin_amdgcn_sched_barrier(0);
       }
       }
@@ -467,7 +489,10 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     const Words ws = keyed(k0, k1, 0u, wctr, TAG_WSEED);
     const float phit = u24(ws.w0);  // phi in turns
     const int seed = (int)(((uint64_t)ws.w2 * (uint64_t)S::V) >> 32);
-    // r_i = cos(theta_i - phi) (src/updates.rs:579-582)
+    // r_i = cos(theta_i - phi) (src/updates.rs:579-582), stored scaled by
+    // sqrt(2 beta log2 e), so that a bond's exponent -2 beta J r_i r_j (in
+    // base 2) is the product of J and the two stored values
+    const float rsc = __builtin_sqrtf(2.0f * beta_l2e);
     if (own) {
       const int zpar = (x + y) & 1;
 #pragma unroll
@@ -477,13 +502,12 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
 #pragma unroll
           for (int k = 0; k < H; ++k) {
             const int z = 2 * k + (c ^ zpar);
-            rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(opq(th[j][c * H + k]) - phit);
+            rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(opq(th[j][c * H + k]) - phit) * rsc;
             __builtin_amdgcn_sched_barrier(0);
           }
     }
     __syncthreads();
     const int wave = tw >> 6, lane = tw & 63;
-    const float m2b = -2.0f * beta_l2e;
     const int xp = (x + 1 == L) ? 0 : x + 1;
     // Bond decisions read r from LDS only, so this loop stays rolled; it takes
     // G sites (z .. z+G-1) of a column per step.  Production stream: the 3G
@@ -527,13 +551,15 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
             wv[1] = w24(w.w1);
             wv[2] = w24(w.w2);
           }
-          const float px = jj.x * ri * rxp, py = jj.x * ri * ryp, pz = jj.y * ri * rzp;
+          // 2 beta log2(e) J r_i r_j of the three bonds (r scaled as stored)
+          const float qxy = jj.x * ri, qz = jj.y * ri;
+          const float px = qxy * rxp, py = qxy * ryp, pz = qz * rzp;
           // active iff p > 0 and u < 1 - exp(-2 beta p), as u 2^24 < (1 - e) 2^24
           // with the right side in one fma.  One compare decides both: for
           // p <= 0, e >= 1 makes the right side <= 0 <= u 2^24.  Its ballot is
           // the compare's lane mask (lanes outside the lattice add no bits).
           auto active = [&](float wu, float pb) {
-            const float e = exp2_hw(m2b * pb);
+            const float e = exp2_hw(-pb);
             return __builtin_amdgcn_ballot_w64(wu < __builtin_fmaf(-e, 0x1.0p24f, 0x1.0p24f));
           };
           gx = active(wv[0], px);
