@@ -128,11 +128,13 @@ SLSF_D uint32_t run_closure(uint32_t n, const RowRuns& r) {
 }
 
 
-// An empty asm makes a value opaque to the optimizer.  Every phase reads its
-// angles through opq(): otherwise the compiler rewrites cos(acc ? t1 : t0) as
-// acc ? cos(t1) : cos(t0) and keeps the Metropolis step's sines and cosines
-// alive from one phase to the next, which the register budget of the
-// 1024-thread L=32 workgroup cannot hold.
+// An empty asm makes a value opaque to the optimizer.  The exchange and the
+// Metropolis step read their angles through opq(): otherwise the compiler
+// rewrites cos(acc ? t1 : t0) as acc ? cos(t1) : cos(t0) and keeps the
+// Metropolis step's sines and cosines alive from one phase to the next, which
+// the register budget of the 1024-thread L=32 workgroup cannot hold.  (The
+// Wolff projection cos(theta - phi) shares nothing with them and reads the
+// angles directly, saving the copy an opq() of a live value costs.)
 SLSF_D float opq(float v) {
   asm("" : "+v"(v));
   return v;
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
 #pragma unroll
           for (int k = 0; k < H; ++k) {
             const int z = 2 * k + (c ^ zpar);
-            rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(opq(th[j][c * H + k]) - phit) * rsc;
+            rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(th[j][c * H + k] - phit) * rsc;
             __builtin_amdgcn_sched_barrier(0);
           }
     }
@@ -640,7 +642,6 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         const int r = tw + i * S::NT;
         if (RPT * S::NT > S::ROWS && r >= S::ROWS) break;
         uint32_t n = mine[i];
-#pragma unroll
         for (int b = 0; b < NB; ++b) n |= *nb_row[i][b] & nb_mask[i][b];
         n = run_closure<L>(n, runs[i]);
         if (n != mine[i]) {
