@@ -524,7 +524,17 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       const int j = jq / L, zq = jq % L;
       const int yj = y + j * S::YS;
       const int yp = (yj + 1 == L) ? 0 : yj + 1;
-      const uint32_t i0 = (uint32_t)(x + L * (yj + L * zq));
+      constexpr int PL = L * L;  // index step of one plane
+      // Per-thread bases of the group, made opaque once: every site of the
+      // group reads its r values, couplings and row words at compile-time
+      // offsets from them.  (Left to itself the compiler merges zq + q into one
+      // index and rebuilds every address of every site.)  bzl is the +z
+      // neighbour of the group's last site, which wraps at the lattice's top.
+      int b0 = x + L * (yj + L * zq), bx = xp + L * (yj + L * zq), by = x + L * (yp + L * zq);
+      int bzl = (zq + BG::G == L) ? x + L * yj : b0 + BG::G * PL;
+      int bj = zq, brow = wave * S::RPW + lane + j * S::YS + L * zq;
+      asm volatile("" : "+v"(b0), "+v"(bx), "+v"(by), "+v"(bzl), "+v"(bj), "+v"(brow));
+      const uint32_t i0 = (uint32_t)b0;
       Words wq[BG::NC];
 #pragma unroll
       for (int q = 0; q < BG::G; ++q) {
@@ -533,16 +543,14 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           for (int c = 0; c < BG::NC; ++c)
             if (BG::first_site(c) == q) wq[c] = keyed(k0, k1, i0, wctr, TAG_WBOND_GROUP + c);
         }
-        const int z = zq + q;
-        const int zn = (z + 1 == L) ? 0 : z + 1;
         unsigned long long gx = 0, gy = 0, gz = 0;  // bond ballots of the wave
         if (own) {
-          const int i = x + L * (yj + L * z);
+          const int i = b0 + q * PL;  // site (x, yj, zq + q)
           const float ri = rbuf[i];
-          const float rxp = rbuf[xp + L * (yj + L * z)];
-          const float ryp = rbuf[x + L * (yp + L * z)];
-          const float rzp = rbuf[x + L * (yj + L * zn)];
-          const float4 jj = jt[z];
+          const float rxp = rbuf[bx + q * PL];
+          const float ryp = rbuf[by + q * PL];
+          const float rzp = (q + 1 < BG::G) ? rbuf[i + PL] : rbuf[bzl];
+          const float4 jj = jt[bj + q];
           float wv[3];  // the bond uniforms' 24-bit integers as floats
           if (PAIRED) {
 #pragma unroll
@@ -569,7 +577,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           gz = active(wv[2], pz);
         }
         if (lane < S::RPW) {
-          const int row = wave * S::RPW + lane + j * S::YS + L * z;
+          const int row = brow + L * q;
           const int sh = lane * S::RW;
           bxm[row] = (uint32_t)(gx >> sh) & S::MASK;
           bym[row] = (uint32_t)(gy >> sh) & S::MASK;
