@@ -140,6 +140,16 @@ SLSF_D float opq(float v) {
   return v;
 }
 
+// An LDS pointer made opaque, kept in the LDS address space: a loop over
+// compile-time offsets from it compiles to ds_* instructions with immediate
+// offsets (up to 64 KiB from the base) and no per-access address arithmetic.
+typedef __attribute__((address_space(3))) float lds_f32;
+SLSF_D lds_f32* lds_base(float* p) {
+  lds_f32* q = (lds_f32*)p;
+  asm("" : "+v"(q));
+  return q;
+}
+
 // The 24-bit integer of a uniform word as a float (exact): u24(w) = w24(w) 2^-24.
 // Comparisons and affine maps of u are rewritten on w24 where the scaling by
 // a power of two stays exact, which saves the multiply and keeps every value.
@@ -500,13 +510,21 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
 #pragma unroll
       for (int j = 0; j < CPT; ++j)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < 2; ++c) {
+          // the column's colour-c sites z = 2k + (c ^ zpar) lie two planes
+          // apart: from two bases (k < H/2, k >= H/2) every offset fits the
+          // 16-bit immediate of ds_write
+          float* col = rbuf + (x + L * (y + j * S::YS + L * (c ^ zpar)));
+          lds_f32* r0 = lds_base(col);
+          lds_f32* r1 = lds_base(col + (H / 2) * 2 * L * L);
 #pragma unroll
           for (int k = 0; k < H; ++k) {
-            const int z = 2 * k + (c ^ zpar);
-            rbuf[x + L * (y + j * S::YS + L * z)] = cos_t(th[j][c * H + k] - phit) * rsc;
+            const float v = cos_t(th[j][c * H + k] - phit) * rsc;
+            if (k < H / 2) r0[2 * k * L * L] = v;
+            else r1[2 * (k - H / 2) * L * L] = v;
             __builtin_amdgcn_sched_barrier(0);
           }
+        }
     }
     __syncthreads();
     const int wave = tw >> 6, lane = tw & 63;
