@@ -264,6 +264,8 @@ struct ResLds {
   uint32_t bxm[S::ROWS], bym[S::ROWS], bzm[S::ROWS], vis[S::ROWS];
   float4 jt[L];  // {J_xy[z], J_z[z], J_z[z-1], 0}
   double red[S::NW * 9];
+  double acc[13];   // the replica's open-bin accumulators for the launch
+  int32_t bst[2];   // and its bin counters (sweeps in the open bin, bins done)
   int flags[3];
   float2 xbuf[S::V / 2];  // opposite-colour (cos, sin); Wolff: r[V], then path masks
 };
@@ -319,6 +321,12 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     jt[t] = make_float4(a.jxy[d.jz_off + t], a.jz[d.jz_off + t], a.jz[d.jz_off + zm], 0.0f);
   }
   if (t < 3) flags[t] = 0;
+  // the replica's measurement state comes in for the launch; a launch with
+  // no measurement state (the autotuner's trials, which only thermalize and
+  // pass no arrays) neither reads nor writes it
+  const bool has_bins = a.accum != nullptr && a.bin_state != nullptr;
+  if (has_bins && t < 13) sm.acc[t] = a.accum[(int64_t)rep * 13 + t];
+  if (has_bins && t < 2) sm.bst[t] = a.bin_state[2 * rep + t];
   __syncthreads();
 
   uint32_t nacc_wave = 0;  // wave-uniform: accepted proposals of this wave
@@ -405,10 +413,14 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         ua = u24(w.w2);
       }
       // 2u - 1 with u = wp 2^-24, in one exact fma
-      const float t1 = fract_hw(t0 + wturn * __builtin_fmaf(wp, 0x1.0p-23f, -1.0f));
+      // explicit fmas here and below: an expression left for the compiler to
+      // contract may be fused differently in different instances of this
+      // kernel (columns per thread), and a replica's chain must not depend on
+      // which instance runs it
+      const float t1 = fract_hw(__builtin_fmaf(wturn, __builtin_fmaf(wp, 0x1.0p-23f, -1.0f), t0));
       const float c0 = cos_t(t0), s0 = sin_t(t0), c1 = cos_t(t1), s1 = sin_t(t1);
       if (meas) measure_site(part, c0, s0, q, nxp, nxm, nyp, nym, nzp, nzm, jj);
-      const float de = (c0 - c1) * hx + (s0 - s1) * hy;
+      const float de = __builtin_fmaf(c0 - c1, hx, (s0 - s1) * hy);
       const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
       const bool down = de <= 0.0f, up = ua < pacc;
       const bool acc = down | up;
@@ -424,29 +436,50 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     }
   };
 
-  // folds the partial sums of one measurement into the replica's FP64 bins
+  // folds the partial sums of one measurement into the replica's FP64 bins:
+  // wave sums (shuffles), then wave 0 finishes with one lane per quantity --
+  // lane k < 9 adds quantity k over the waves in wave order, lane k < 13
+  // accumulates series k -- against the accumulators held in LDS for the
+  // launch, so the other waves never wait on a global memory round trip
   auto measure_finish = [&]() __attribute__((always_inline)) {
     double v[9] = {-(double)(part.ec[0] + part.ec[1] + part.ec[2]), (double)part.mx,
                    (double)part.my, (double)part.ec[0], (double)part.ec[1], (double)part.ec[2],
                    (double)part.es[0], (double)part.es[1], (double)part.es[2]};
-    wg_sum<9, S::NW>(v, red);
-    if (t == 0) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if ((t & 63) == 0)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) red[(t >> 6) * 9 + k] = v[k];
+    __syncthreads();
+    if (t < 64) {
+      double s = 0.0;
+      if (t < 9)
+        for (int w = 0; w < S::NW; ++w) s += red[w * 9 + t];
+      const double s0 = __shfl(s, 0, 64), s1 = __shfl(s, 1, 64), s2 = __shfl(s, 2, 64);
+      // series: e, e^2, m^2, m^4, cos_x..z, sin_x..z, sin_x^2..sin_z^2
+      const double sk = __shfl(s, t < 4 ? 0 : (t < 10 ? t - 1 : t - 4), 64);
       const double vol = (double)S::V;
-      const double e = v[0] / vol;
-      const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
-      const double vals[13] = {e,    e * e, m2,   m2 * m2,     v[3],        v[4],       v[5],
-                               v[6], v[7],  v[8], v[6] * v[6], v[7] * v[7], v[8] * v[8]};
-      double* acc = a.accum + (int64_t)rep * 13;
-      for (int k = 0; k < 13; ++k) acc[k] += vals[k];
-      int32_t* st = a.bin_state + 2 * rep;
-      st[0] += 1;
-      if (st[0] == d.binsize) {
-        if (st[1] < a.bin_cap)
-          for (int k = 0; k < 13; ++k)
-            a.bins[((int64_t)rep * 13 + k) * a.bin_cap + st[1]] = acc[k] / (double)d.binsize;
-        for (int k = 0; k < 13; ++k) acc[k] = 0.0;
-        st[0] = 0;
-        st[1] += 1;
+      const double e = s0 / vol;
+      const double m2 = __builtin_fma(s1, s1, s2 * s2) / (vol * vol);
+      const double val = t == 0   ? e
+                         : t == 1 ? e * e
+                         : t == 2 ? m2
+                         : t == 3 ? m2 * m2
+                         : t < 10 ? sk
+                                  : sk * sk;
+      const int cnt = sm.bst[0] + 1, done = sm.bst[1];
+      const bool full = cnt == d.binsize;
+      if (t < 13) {
+        const double acc = sm.acc[t] + val;
+        if (full && done < a.bin_cap)
+          a.bins[((int64_t)rep * 13 + t) * a.bin_cap + done] = acc / (double)d.binsize;
+        sm.acc[t] = full ? 0.0 : acc;
+      }
+      if (t == 0) {
+        sm.bst[0] = full ? 0 : cnt;
+        sm.bst[1] = full ? done + 1 : done;
       }
     }
   };
@@ -752,6 +785,8 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
           a.cs[g] = make_float2(cos_t(v), sin_t(v));
         }
   }
+  if (has_bins && t < 13) a.accum[(int64_t)rep * 13 + t] = sm.acc[t];
+  if (has_bins && t < 2) a.bin_state[2 * rep + t] = sm.bst[t];
   double tallies[2] = {(t & 63) == 0 ? (double)nacc_wave : 0.0, (double)ncluster};
   __syncthreads();
   wg_sum<2, S::NW>(tallies, red);
