@@ -442,27 +442,29 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   // accumulates series k -- against the accumulators held in LDS for the
   // launch, so the other waves never wait on a global memory round trip
   auto measure_finish = [&]() __attribute__((always_inline)) {
-    double v[9] = {-(double)(part.ec[0] + part.ec[1] + part.ec[2]), (double)part.mx,
-                   (double)part.my, (double)part.ec[0], (double)part.ec[1], (double)part.ec[2],
-                   (double)part.es[0], (double)part.es[1], (double)part.es[2]};
+    // quantities: mx, my, cos_x..z, sin_x..z (the energy is -(cos_x + cos_y +
+    // cos_z), formed from their FP64 totals)
+    double v[8] = {(double)part.mx,    (double)part.my,    (double)part.ec[0], (double)part.ec[1],
+                   (double)part.ec[2], (double)part.es[0], (double)part.es[1], (double)part.es[2]};
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
+    for (int k = 0; k < 8; ++k)
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
     if ((t & 63) == 0)
 #pragma unroll
-      for (int k = 0; k < 9; ++k) red[(t >> 6) * 9 + k] = v[k];
+      for (int k = 0; k < 8; ++k) red[(t >> 6) * 8 + k] = v[k];
     __syncthreads();
     if (t < 64) {
       double s = 0.0;
-      if (t < 9)
-        for (int w = 0; w < S::NW; ++w) s += red[w * 9 + t];
-      const double s0 = __shfl(s, 0, 64), s1 = __shfl(s, 1, 64), s2 = __shfl(s, 2, 64);
+      if (t < 8)
+        for (int w = 0; w < S::NW; ++w) s += red[w * 8 + t];
+      const double s0 = __shfl(s, 0, 64), s1 = __shfl(s, 1, 64);
+      const double c0 = __shfl(s, 2, 64), c1 = __shfl(s, 3, 64), c2 = __shfl(s, 4, 64);
       // series: e, e^2, m^2, m^4, cos_x..z, sin_x..z, sin_x^2..sin_z^2
-      const double sk = __shfl(s, t < 4 ? 0 : (t < 10 ? t - 1 : t - 4), 64);
+      const double sk = __shfl(s, t < 10 ? t - 2 : t - 5, 64);
       const double vol = (double)S::V;
-      const double e = s0 / vol;
-      const double m2 = __builtin_fma(s1, s1, s2 * s2) / (vol * vol);
+      const double e = -(c0 + c1 + c2) / vol;
+      const double m2 = __builtin_fma(s0, s0, s1 * s1) / (vol * vol);
       const double val = t == 0   ? e
                          : t == 1 ? e * e
                          : t == 2 ? m2
