@@ -385,6 +385,12 @@ typedef struct {
 } slsf_ceilings;
 int slsf_measure_ceilings(slsf_ceilings* out);
 
+/* Device self-check: the number of floats x in {-0} U [+0, +inf] for which the
+ * hardware v_exp_f32 (2^x) returns less than 1, over all of them.  The FP32
+ * resident kernel decides "dE <= 0 or u < exp(-beta dE)" (src/updates.rs:255-264)
+ * by the second test alone, which is exact only when this count is 0. */
+int slsf_check_exp2_floor(uint64_t* below);
+
 /* ======================================================================== */
 /* Job configuration (TOML) and task expansion                               */
 /* ======================================================================== */

@@ -55,6 +55,20 @@ __global__ void __launch_bounds__(CT) k_issue(float* out, float seed, int iters)
   out[blockIdx.x * CT + threadIdx.x] = s;
 }
 
+// v_exp_f32 of every float x in {-0} U [+0, +inf]: counts the results below 1.
+// The resident kernel's Metropolis test u < exp(-beta dE) (u < 1) also decides
+// the dE <= 0 case only if there are none.
+__global__ void k_exp2_floor(unsigned long long* below) {
+  const uint64_t n = 0x7F800001ull;  // bit patterns +0 .. +inf, then -0
+  unsigned long long local = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t bits = i == n ? 0x80000000u : (uint32_t)i;
+    if (!(__builtin_amdgcn_exp2f(__uint_as_float(bits)) >= 1.0f)) ++local;
+  }
+  if (local) atomicAdd(below, local);
+}
+
 }  // namespace
 
 using slsf::clear_error;
@@ -106,5 +120,26 @@ extern "C" int slsf_measure_ceilings(slsf_ceilings* out) {
   out->exp_wave_instr_per_s = lanes * issue_iters * CHAINS / 64.0 / t_exp;
   out->compute_units = prop.multiProcessorCount;
   out->reserved = 0;
+  return SLSF_OK;
+}
+
+extern "C" int slsf_check_exp2_floor(uint64_t* below) {
+  clear_error();
+  if (!below) return fail(SLSF_E_INVALID, "null argument");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(SLSF_E_DEVICE, "no usable GPU (hipGetDeviceCount found none)");
+  unsigned long long* d = nullptr;
+  if (hipMalloc(&d, sizeof *d) != hipSuccess) return fail(SLSF_E_DEVICE, "hipMalloc failed");
+  unsigned long long h = 0;
+  hipError_t err = hipMemset(d, 0, sizeof *d);
+  if (err == hipSuccess) {
+    k_exp2_floor<<<4096, 256>>>(d);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (err != hipSuccess) return fail(SLSF_E_DEVICE, hipGetErrorString(err));
+  *below = h;
   return SLSF_OK;
 }

@@ -422,12 +422,12 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
       if (meas) measure_site(part, c0, s0, q, nxp, nxm, nyp, nym, nzp, nzm, jj);
       const float de = __builtin_fmaf(c0 - c1, hx, (s0 - s1) * hy);
       const float pacc = exp2_hw(-de * beta_l2e);  // evaluated always: no branch
-      const bool down = de <= 0.0f, up = ua < pacc;
-      const bool acc = down | up;
-      // scalar count per wave; the ballots of the two compares are their
-      // lane masks as they stand (a ballot of the combined bool would first
-      // move it through a vector register)
-      nacc_wave += __popcll(__builtin_amdgcn_ballot_w64(down) | __builtin_amdgcn_ballot_w64(up));
+      // accept iff de <= 0 or u < exp(-beta de) (src/updates.rs:255-264).  With
+      // u < 1 the second test alone decides both: v_exp_f32 returns at least 1
+      // for every argument >= 0 (checked over all such floats on the device).
+      // The ballot of the one compare is its lane mask: a scalar count per wave.
+      const bool acc = ua < pacc;
+      nacc_wave += __popcll(__builtin_amdgcn_ballot_w64(acc));
       th[j][p * H + k] = acc ? t1 : t0;
       // one site in flight per thread: keeps the register budget of the
       // 1024-thread L=32 workgroup (occupancy supplies the latency hiding)

@@ -201,6 +201,7 @@ _SIGS = {
     "slsf_server_slot_bytes": (ctypes.c_int64, [ctypes.c_int32]),
     "slsf_device_memory": (ctypes.c_int, [_i64p, _i64p]),
     "slsf_measure_ceilings": (ctypes.c_int, [ctypes.c_void_p]),
+    "slsf_check_exp2_floor": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64)]),
     "slsf_run_theta_simulation": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, ctypes.c_double, _dp, _dp,
@@ -495,6 +496,13 @@ def measure_ceilings() -> dict:
     c = Ceilings()
     _check(_lib.slsf_measure_ceilings(ctypes.byref(c)))
     return {n: getattr(c, n) for n, _ in Ceilings._fields_ if n != "reserved"}
+
+
+def check_exp2_floor() -> int:
+    """Floats x >= 0 (and -0) for which the hardware 2^x returns less than 1."""
+    n = ctypes.c_uint64()
+    _check(_lib.slsf_check_exp2_floor(ctypes.byref(n)))
+    return n.value
 
 
 def device_memory() -> tuple[int, int]:

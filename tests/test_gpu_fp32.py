@@ -379,3 +379,11 @@ def test_large_lattice_invariants(precision):
     assert len(e) == 3 and np.all(np.isfinite(e))
     tol = 2e-5 if precision == "fp32" else 1e-10
     assert abs(e[-1] - o["energy"]) < tol * abs(o["energy"]) + 1e-6
+
+
+def test_hardware_exp2_is_at_least_one_for_nonnegative_arguments():
+    """The resident kernel accepts a Metropolis move iff u < exp(-beta dE),
+    which also covers dE <= 0 (src/updates.rs:255-264) because u < 1 and the
+    hardware 2^x returns at least 1 for every x >= 0: checked here over all
+    such floats (and -0)."""
+    assert slsf.check_exp2_floor() == 0
