@@ -12,8 +12,8 @@
 // Metropolis, colour p (src/updates.rs:440-486): all threads write the
 // (cos, sin) of their opposite-colour sites into LDS slot idx/2 (every pair of
 // x-neighbours holds one site of each colour), barrier, then every thread
-// updates its colour-p sites reading the 4 in-plane neighbours from LDS and the
-// 2 z-neighbours from its own registers.
+// updates its colour-p sites reading all six neighbours' (cos, sin) from LDS
+// (the two z-neighbours are its own column's sites, exchanged with the rest).
 //
 // Wolff (src/updates.rs:551-620): r_i = cos(theta_i - phi) for every site goes
 // to LDS, then every site decides its three canonical bonds (+x, +y, +z) with
