@@ -37,12 +37,14 @@ void set_extents(GlobalState& g, const RepDev* reps, int n) {
   g.max_lz = 0;
   g.obs_blocks = 0;
   g.max_tiles = 0;
+  g.max_tiles8 = 0;
   for (int r = 0; r < n; ++r) {
     g.max_volume = std::max<int64_t>(g.max_volume, reps[r].volume);
     g.max_plane = std::max<int64_t>(g.max_plane, (int64_t)reps[r].lx * reps[r].ly);
     g.max_lz = std::max(g.max_lz, reps[r].lz);
     g.obs_blocks = std::max(g.obs_blocks, obs_blocks_of(reps[r].lx, reps[r].ly, reps[r].lz));
-    g.max_tiles = std::max(g.max_tiles, wolff_tiles_of(reps[r].lx, reps[r].ly, reps[r].lz));
+    g.max_tiles = std::max(g.max_tiles, wolff_tiles_of<16>(reps[r].lx, reps[r].ly, reps[r].lz));
+    g.max_tiles8 = std::max(g.max_tiles8, wolff_tiles_of<8>(reps[r].lx, reps[r].ly, reps[r].lz));
   }
 }
 

@@ -20,7 +20,8 @@ struct GlobalState {
   void* jz = nullptr;
   int max_lz = 0;           // largest replica extents (grid sizes)
   int64_t max_plane = 0;
-  int max_tiles = 0;        // largest wolff_tiles_of
+  int max_tiles = 0;        // largest wolff_tiles_of<16>
+  int max_tiles8 = 0;       // largest wolff_tiles_of<8> (the shallow tiles of small launches)
   uint8_t* bondmask = nullptr;        // [total_sites] active-bond bits of the base site
   uint32_t* visited = nullptr;        // [total_sites / 32] cluster membership bitmap
   uint32_t* label = nullptr;          // [total_sites] union-find forest of the cluster step
