@@ -107,6 +107,28 @@ def test_chain_c1_bitexact():
         assert abs(og[k] - oo[k]) <= 1e-12 * max(1.0, abs(oo[k])), k
 
 
+def test_cluster_tiling_does_not_change_the_chain():
+    """The cluster kernels take 16x16x8 tiles when a launch has fewer 16-deep
+    tiles than the device has compute units (a replica alone) and 16x16x16
+    tiles otherwise (here five L=64 replicas: 320 tiles).  The cluster is the
+    seed's component either way, so replica 0 follows the same FP64 chain in
+    both, and that chain is the oracle's."""
+    L, T, seed = 64, 1.3, 21
+    jxy = uniform_disorder(L, 4)
+    jz = np.ones(L)
+    one = slsf.Batch([slsf.Replica(L, L, L, T, jxy, jz, seed, np.pi, 1)])
+    many = slsf.Batch([slsf.Replica(L, L, L, T, jxy, jz, seed + i, np.pi, 1) for i in range(5)])
+    for b in (one, many):
+        b.init_angles()
+        b.sweep(4)
+    assert np.array_equal(one.get_theta(0), many.get_theta(0))
+    _, lat = make_pair(L, L, L, T, seed, jxy, jz)
+    for s in range(4):
+        lat.metro_replay(T, np.pi, seed, s)
+        lat.wolff_replay(T, seed, s)
+    assert np.array_equal(one.get_theta(0), lat.theta)
+
+
 def test_batch_matches_each_replica_alone():
     """Replicas of different shapes, temperatures, seeds and wolff_steps in one
     batch follow exactly the chains they follow alone."""
