@@ -271,7 +271,7 @@ def bench_c3(args, slsf):
         p.close()
         plain = V * nplain / sp["seconds"]
 
-    spl = args.sweeps_per_launch or 50
+    spl = args.sweeps_per_launch or 1000  # the library's default launch length
     kernel_ms = st.get("kernel_ms", 0.0)  # HIP events on the launch stream
     launches = max(st["launches"], 1)
     bytes_per_update = 8.0  # FP32 angle read + write per site update
