@@ -64,48 +64,60 @@ void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s
 // k takes blocks k, k+64, ... in order, then a fixed shuffle tree), forms the
 // 13 series values and appends a bin mean every `binsize` measurements.  A
 // replica is measured iff it takes part in this sweep and its counter (not yet
-// advanced) is at least measure_from.
+// advanced) is at least measure_from.  With rounds >= 0 the same block then does
+// the replica's k_sweep_end(metro = 1, rounds) bookkeeping, which saves a
+// launch on measured sweeps.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64)
-k_accumulate(const RepDev* __restrict__ reps, const double* __restrict__ partials,
-             int obs_stride, double* accum, double* bins, int32_t* bin_state, int32_t bin_cap) {
+k_accumulate(RepDev* reps, const double* __restrict__ partials, int obs_stride, double* accum,
+             double* bins, int32_t* bin_state, int32_t bin_cap, int rounds) {
   const int r = blockIdx.x;
-  const RepDev& d = reps[r];
-  if (!d.active || d.metro_ctr < d.measure_from) return;
+  RepDev& d = reps[r];
   const int lane = threadIdx.x;
-  const int nb = obs_blocks_of(d.lx, d.ly, d.lz);
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int b = lane; b < nb; b += 64)
+  const bool measured = d.active && d.metro_ctr >= d.measure_from;
+  if (measured) {
+    const int nb = obs_blocks_of(d.lx, d.ly, d.lz);
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = lane; b < nb; b += 64)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) v[k] += partials[((int64_t)r * obs_stride + b) * 9 + k];
+      for (int k = 0; k < 9; ++k) v[k] += partials[((int64_t)r * obs_stride + b) * 9 + k];
 #pragma unroll
-  for (int k = 0; k < 9; ++k)
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-  if (lane != 0) return;
-  const int32_t binsize = d.binsize;
-  const double vol = (double)d.volume;
-  const double e = v[0] / vol;
-  const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
-  const double vals[13] = {e,    e * e, m2,   m2 * m2,     v[3],        v[4],       v[5],
-                           v[6], v[7],  v[8], v[6] * v[6], v[7] * v[7], v[8] * v[8]};
-  double* acc = accum + (int64_t)r * 13;
-  for (int k = 0; k < 13; ++k) acc[k] += vals[k];
-  int32_t* st = bin_state + 2 * r;
-  st[0] += 1;
-  if (st[0] == binsize) {
-    if (st[1] < bin_cap)
-      for (int k = 0; k < 13; ++k)
-        bins[((int64_t)r * 13 + k) * bin_cap + st[1]] = acc[k] / (double)binsize;
-    for (int k = 0; k < 13; ++k) acc[k] = 0.0;
-    st[0] = 0;
-    st[1] += 1;
+    for (int k = 0; k < 9; ++k)
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    if (lane == 0) {
+      const int32_t binsize = d.binsize;
+      const double vol = (double)d.volume;
+      const double e = v[0] / vol;
+      const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
+      const double vals[13] = {e,    e * e, m2,   m2 * m2,     v[3],        v[4],       v[5],
+                               v[6], v[7],  v[8], v[6] * v[6], v[7] * v[7], v[8] * v[8]};
+      double* acc = accum + (int64_t)r * 13;
+      for (int k = 0; k < 13; ++k) acc[k] += vals[k];
+      int32_t* st = bin_state + 2 * r;
+      st[0] += 1;
+      if (st[0] == binsize) {
+        if (st[1] < bin_cap)
+          for (int k = 0; k < 13; ++k)
+            bins[((int64_t)r * 13 + k) * bin_cap + st[1]] = acc[k] / (double)binsize;
+        for (int k = 0; k < 13; ++k) acc[k] = 0.0;
+        st[0] = 0;
+        st[1] += 1;
+      }
+    }
+  }
+  if (rounds >= 0 && lane == 0) {  // as k_sweep_end(metro = 1, rounds), after every read of d
+    if (d.active) {
+      d.metro_ctr += 1u;
+      d.wolff_ctr += (uint32_t)(rounds < d.wolff_steps ? rounds : d.wolff_steps);
+    }
+    d.active = d.metro_ctr < d.stop_at ? 1u : 0u;
   }
 }
 
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t bin_cap, hipStream_t s) {
+                       int32_t bin_cap, int rounds, hipStream_t s) {
   k_accumulate<<<g.nrep, 64, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins, bin_state,
-                                     bin_cap);
+                                     bin_cap, rounds);
 }
 
 // ---------------------------------------------------------------------------

@@ -39,8 +39,9 @@ template <typename R> void launch_init(const GlobalState& g, int cold, hipStream
 template <typename R> void launch_refresh(const GlobalState& g, hipStream_t s);
 // A sweep of the HBM path: launch_sweep_end(g, 0, 0) marks the replicas whose
 // window is open, then per sweep launch_metro_phases, launch_wolff for rounds
-// 0..wmax-1, optionally launch_observables + launch_accumulate, and
-// launch_sweep_end(g, 1, wmax) advances the counters and marks the next sweep.
+// 0..wmax-1, then either launch_observables + launch_accumulate(.., wmax, ..)
+// or launch_sweep_end(g, 1, wmax), which advance the counters and mark the
+// next sweep.
 template <typename R> void launch_metro_phases(const GlobalState& g, uint8_t* decisions,
                                                hipStream_t s);
 // cluster round `round` (replicas with wolff_steps <= round sit it out): seed,
@@ -57,8 +58,10 @@ template <typename R> void launch_corr(const GlobalState& g, int replica, int64_
                                        double* out_xy, double* out_z, hipStream_t s);
 // zeroes the accepted-move and cluster-site tallies
 void launch_clear_counters(const GlobalState& g, hipStream_t s);
+// folds a measured sweep into the bins; with rounds >= 0 also advances the
+// counters as launch_sweep_end(g, 1, rounds) would
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t bin_cap, hipStream_t s);
+                       int32_t bin_cap, int rounds, hipStream_t s);
 
 // The FP64 replay of one small replica inside one workgroup
 // (k_replay_resident): a chunk of `nsweeps` sweeps of the plain driver, with
