@@ -40,6 +40,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+LDS_PEAK_NOMINAL_GBS = 78643.2  # 256 CUs x 128 B/clk x 2.4 GHz
 METRIC = "Metropolis+Wolff site-updates/s (batched vs plain, 1 GPU); job p50/p99 latency"
 DATA = ("synthetic: J_xy[z] ~ U[0.6,1.8] per layer from the realization seed, J_perp=1.0, "
         "random initial angles")
@@ -89,28 +90,53 @@ def c3_replicas(slsf, n_real=256, temps=None, L=32, wolff_steps=1):
     return disorder_batch(slsf, [L], temps, n_real, 1000, wolff_steps)[0]
 
 
-def cpu_baseline(budget_s=12.0):
-    """Reference-mode oracle (SplitMix64 stream, red/black with 8-lane batches,
-    Wolff DFS with trig cache, libm sincos/exp) on one L=32 replica per core."""
+def cpu_cores():
+    """Host cores the CPU baseline may use: the CPUs this process may run on,
+    capped by OMP_NUM_THREADS when set (a GPU box allots its GPU a share of the
+    host's cores and states it there)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_sample_plan(budget_s=12.0):
+    """Sweeps per replica of the CPU sample: about budget_s of work on each core,
+    calibrated on this host.  The first fifth thermalizes; the rest is measured
+    every sweep in bins of up to 100 measurements (at least 20 bins)."""
+    rate1 = _cpu_worker((0, 3, 0, 1))[0]  # calibrate one core
+    sweeps = max(500, int(budget_s * rate1 / (32 ** 3)))
+    therm = sweeps // 5
+    measured = sweeps - therm
+    binsize = max(1, min(100, measured // 20))
+    return therm, measured, binsize
+
+
+def cpu_baseline(plan):
+    """Reference-mode oracle (SplitMix64 stream, red/black with 8- or 4-lane
+    batches and vector sincos/exp as the host's width allows, Wolff DFS with
+    trig cache) on one L=32 replica per core."""
     import multiprocessing as mp
-    cores = min(os.cpu_count() or 1, 16)
-    rate1 = _cpu_worker((0, 3))[0]  # calibrate one core
-    sweeps = max(2, int(budget_s * rate1 / (32 ** 3)))
+    import oracle_lib as orc
+    therm, measured, binsize = plan
+    cores = cpu_cores()
+    sweeps = therm + measured
     t0 = time.perf_counter()
     with mp.get_context("fork").Pool(cores) as pool:
-        out = pool.map(_cpu_worker, [(i, sweeps) for i in range(cores)])
+        out = pool.map(_cpu_worker, [(i, sweeps, therm, binsize) for i in range(cores)])
     wall = time.perf_counter() - t0
     total = cores * sweeps * 32 ** 3
-    therm = sweeps // 5
+    lanes = orc.host_batch_lanes()
     return {"value": total / wall, "unit": "site-updates/s", "cores": cores,
             "kind": "port",
             "sample": f"{cores} independent L=32 replicas (realizations 0..{cores - 1} of the "
                       f"batched workload at T=1.3, one per core), {sweeps} Metropolis+Wolff "
                       f"sweeps each ({therm} thermalization, then measured every sweep), FP64, "
-                      "reference draw order",
+                      f"reference draw order, {lanes}-lane batches with vector sincos/exp",
             "per_core": float(np.mean([o[0] for o in out])),
             "cpu_model": _cpu_model(), "smt_active": _smt_active(),
-            "host_cpus": os.cpu_count(),
+            "host_cpus": os.cpu_count(), "batch_lanes": lanes,
             "bins": {i: o[1] for i, o in enumerate(out)}}
 
 
@@ -134,23 +160,23 @@ def _smt_active():
 
 
 def _cpu_worker(arg):
-    """One replica: `sweeps` reference-mode sweeps, the last 4/5 measured every
-    sweep (the 13 series, binned in bins of up to 100 measurements)."""
-    idx, sweeps = arg
+    """One replica: `sweeps` reference-mode sweeps, those after the first
+    `therm` measured every sweep (the 13 series, binned by `binsize`)."""
+    idx, sweeps, therm, binsize = arg
     import ctypes
     import oracle_lib as orc
     L = orc.load(fast=True)
+    batch8 = orc.host_batch_lanes() == 8
     lat = orc.Lattice(32, 32, 32, lib_=L)
     lat.set_couplings(splitmix_uniform_layers(32, 1000 + idx), np.ones(32))
     r = orc.FastRng()
     L.or_rng_seed(ctypes.byref(r), 7 + idx)
     L.or_initialize_angles(lat.ref, 0, ctypes.byref(r))
     lat.refresh(False)
-    therm = sweeps // 5
     rows = []
     t0 = time.perf_counter()
     for k in range(sweeps):
-        lat.metro_ref(1.3, np.pi, r, True)
+        lat.metro_ref(1.3, np.pi, r, batch8)
         lat.wolff_ref(1.3, r, True)
         if k >= therm:
             o = lat.observables()
@@ -159,26 +185,34 @@ def _cpu_worker(arg):
             sn = [o["sin_x"], o["sin_y"], o["sin_z"]]
             rows.append([e, e * e, m2, m2 * m2] + c + sn + [v * v for v in sn])
     rate = sweeps * 32 ** 3 / (time.perf_counter() - t0)
+    if not rows:
+        return rate, None
     series = np.array(rows).T  # 13 x measurements, in the order of the GPU series
-    binsize = max(1, min(100, series.shape[1] // 20))
     nb = series.shape[1] // binsize
     bins = series[:, :nb * binsize].reshape(13, nb, binsize).mean(axis=2)
     return rate, bins
 
 
 def fp64_agreement(slsf, gpu_bins, cpu_bins, volume, beta):
-    """FP32 production bins of the batched run against the FP64 CPU runs of
-    the same couplings and temperature (independent chains): per observable
-    the z-scores (gpu - cpu) / sqrt(err_gpu^2 + err_cpu^2) over the replicas,
-    and the disorder average of the differences with its combined error."""
+    """FP32 production bins against FP64 CPU runs of the same couplings and
+    temperature (independent chains): per observable the z-scores
+    (gpu - cpu) / sqrt(err_gpu^2 + err_cpu^2) over the replicas, and the
+    disorder average of the differences with its combined error.  Replicas
+    with fewer than 2 bins on either side have no error bar and are left out;
+    with none left the comparison is None."""
     names = ["Energy", "RhoXY", "RhoZ", "Binder", "Chi"]
-    out = {"replicas": len(cpu_bins), "reference": "oracle reference mode, FP64, CPU",
+    usable = [i for i, cb in cpu_bins.items()
+              if cb is not None and cb.shape[1] >= 2 and i in gpu_bins
+              and gpu_bins[i].shape[1] >= 2]
+    if not usable:
+        return None
+    out = {"replicas": len(usable), "reference": "oracle reference mode, FP64, CPU",
            "observables": {}}
     for name in names:
         z, d, var = [], [], 0.0
-        for i, cb in cpu_bins.items():
+        for i in usable:
             g = slsf.evaluate(gpu_bins[i], volume, beta)[name]
-            c = slsf.evaluate(cb, volume, beta)[name]
+            c = slsf.evaluate(cpu_bins[i], volume, beta)[name]
             err = math.hypot(g[1], c[1])
             z.append((g[0] - c[0]) / err if err > 0 else 0.0)
             d.append(g[0] - c[0])
@@ -192,6 +226,12 @@ def fp64_agreement(slsf, gpu_bins, cpu_bins, volume, beta):
             "mean_difference": float(np.mean(d)),
             "mean_difference_error": math.sqrt(var) / n}
     return out
+
+
+def headline_binsize(steps, requested):
+    """Bin length of the timed run: the requested one, shortened so that the
+    timed sweeps fill at least 20 bins (at least 1)."""
+    return max(1, min(requested, steps // 20))
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -222,31 +262,51 @@ def line(value, unit, steps, warmup, ms_per_step, workload, dtype, higher=True, 
 
 def profile_record(spl, nrep):
     """The committed counter run of k_resident (HBM bytes and VALU instructions
-    per launch), when that run used the same launch shape; None otherwise."""
+    per launch) for launches of `spl` sweeps over `nrep` L=32 replicas; None when
+    no committed run has that launch shape."""
     try:
         with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
+            recs = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("sweeps_per_launch") != spl or t.get("replicas") != nrep:
+    for t in recs if isinstance(recs, list) else [recs]:
+        if t.get("sweeps_per_launch") == spl and t.get("replicas") == nrep:
+            return t
+    return None
+
+
+def instruction_record(nrep):
+    """VALU instructions per update and SIMD busy share of k_resident from any
+    committed counter run over `nrep` L=32 replicas (they do not depend on the
+    launch length)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            recs = json.load(f)
+    except (OSError, ValueError):
         return None
-    return t
+    for t in recs if isinstance(recs, list) else [recs]:
+        if t.get("replicas") == nrep and t.get("valu_per_update"):
+            return t
+    return None
 
 
 # ---------------------------------------------------------------------------
 def bench_c3(args, slsf):
     reps = c3_replicas(slsf, args.realizations)
     V = 32 ** 3
+    binsize = headline_binsize(args.steps, args.binsize)
     b = slsf.Batch(reps, slsf.FP32)
     b.init_angles()
     b.run(args.warmup, 0, 1, sweeps_per_launch=args.sweeps_per_launch)
-    st = b.run(0, args.steps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
-    # bins of realizations 0..15 at T = 1.30 (index 4 of 8), for the comparison
-    # with the FP64 CPU runs of the same couplings
-    gpu_bins = {r: b.all_bins(8 * r + 4) for r in range(min(16, args.realizations))}
+    # the timed region: exactly args.steps sweeps of every replica, with the
+    # measurement of every sweep binned on the device; run() returns after the
+    # batch's stream has drained, and the warmup run before it has too
+    t0 = time.perf_counter()
+    st = b.run(0, args.steps, binsize, sweeps_per_launch=args.sweeps_per_launch)
+    wall = time.perf_counter() - t0
     b.close()
     updates = len(reps) * V * args.steps
-    value = updates / st["seconds"]
+    value = updates / wall
 
     # Metropolis-only rate: the same batch with no cluster step; the Wolff
     # share is the difference of the two times per sweep
@@ -254,7 +314,8 @@ def bench_c3(args, slsf):
     m = slsf.Batch(c3_replicas(slsf, args.realizations, wolff_steps=0), slsf.FP32)
     m.init_angles()
     m.run(min(args.warmup, 200), 0, 1, sweeps_per_launch=args.sweeps_per_launch)
-    sm = m.run(0, msteps, args.binsize, sweeps_per_launch=args.sweeps_per_launch)
+    sm = m.run(0, msteps, headline_binsize(msteps, args.binsize),
+               sweeps_per_launch=args.sweeps_per_launch)
     m.close()
     t_full = st["seconds"] / args.steps
     t_metro = sm["seconds"] / msteps
@@ -271,55 +332,83 @@ def bench_c3(args, slsf):
         p.close()
         plain = V * nplain / sp["seconds"]
 
-    spl = args.sweeps_per_launch or 1000  # the library's default launch length
-    kernel_ms = st.get("kernel_ms", 0.0)  # HIP events on the launch stream
+    # the dominant kernel: k_resident, timed by HIP events on its launch stream
     launches = max(st["launches"], 1)
-    bytes_per_update = 8.0  # FP32 angle read + write per site update
-    alg_bytes_launch = len(reps) * V * bytes_per_update * spl
+    kernel_ms = st.get("kernel_ms", 0.0)
     avg_launch_ms = kernel_ms / launches
-    achieved = (len(reps) * V * args.steps * bytes_per_update / (kernel_ms * 1e-3) / 1e9
-                if kernel_ms > 0 else value * bytes_per_update / 1e9)
+    spl = -(-args.steps // launches)  # sweeps per launch of the timed run
+    per_launch_updates = len(reps) * V * spl
+    kernel_rate = updates / (kernel_ms * 1e-3) if kernel_ms > 0 else value
     prof = profile_record(spl, len(reps))
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS,
-                "traffic": (prof["fetch_bytes"] + prof["write_bytes"]) if prof else None}
-    # the on-chip ceilings the resident kernel actually runs against, measured
-    # on this device: LDS bandwidth (32 B of LDS traffic per site update in the
-    # LDS form) and VALU issue (instructions per update from the counter run)
     ceil = slsf.measure_ceilings()
-    kernel_rate = (len(reps) * V * args.steps / (kernel_ms * 1e-3)) if kernel_ms > 0 else value
-    lds = {"bytes_per_update": 32, "achieved_GBs": kernel_rate * 32 / 1e9,
-           "peak_measured_GBs": ceil["lds_bytes_per_s"] / 1e9,
-           "frac": kernel_rate * 32 / ceil["lds_bytes_per_s"]}
+    # LDS form of the roofline: 32 B of LDS traffic per site update (6 neighbour
+    # reads, the site's own read and its write, FP32), against the nominal LDS
+    # bandwidth of the part; the measured figure beside it
+    lds_bytes = 32.0
+    achieved = kernel_rate * lds_bytes / 1e9
+    roofline = {"bound": "lds", "achieved": achieved, "peak": LDS_PEAK_NOMINAL_GBS,
+                "unit": "GB/s", "frac": achieved / LDS_PEAK_NOMINAL_GBS,
+                "traffic": (prof["fetch_bytes"] + prof["write_bytes"]) if prof else None,
+                "bytes_per_update": lds_bytes,
+                "algorithmic_bytes_per_launch": per_launch_updates * lds_bytes,
+                "peak_measured": ceil["lds_bytes_per_s"] / 1e9,
+                "frac_of_measured": kernel_rate * lds_bytes / ceil["lds_bytes_per_s"]}
+    # the binding ceiling: vector instruction issue (instructions per update
+    # from the committed counter run of this kernel over the same replicas)
+    ins = instruction_record(len(reps))
     valu = None
-    if prof and prof.get("valu_per_update"):
-        issue = kernel_rate * prof["valu_per_update"] / 64.0
-        valu = {"instr_per_update": prof["valu_per_update"], "wave_instr_per_s": issue,
+    if ins:
+        issue = kernel_rate * ins["valu_per_update"] / 64.0
+        valu = {"instr_per_update": ins["valu_per_update"], "wave_instr_per_s": issue,
                 "fma_issue_measured": ceil["fma_wave_instr_per_s"],
                 "frac": issue / ceil["fma_wave_instr_per_s"],
-                # share of the SIMDs' cycles with a vector instruction issued, from
-                # the counter run (SQ_ACTIVE_INST_VALU over SQ_WAVE_CYCLES, 4 waves
-                # per SIMD for the whole launch)
-                "simd_valu_busy": prof.get("simd_valu_busy")}
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
-    agreement = None
-    if cpu is not None:
-        agreement = fp64_agreement(slsf, gpu_bins, cpu.pop("bins"), V, 1.0 / 1.3)
-    line(value, "site-updates/s", args.steps, args.warmup, 1e3 * st["seconds"] / args.steps,
+                # share of the SIMDs' cycles with a vector instruction issued
+                # (SQ_ACTIVE_INST_VALU over SQ_WAVE_CYCLES, 4 waves per SIMD)
+                "simd_valu_busy": ins.get("simd_valu_busy"),
+                "source": ins.get("command")}
+    # the HBM-equivalent figure: 8 B per update (the angle's read and write),
+    # what the same rate would move if every sweep streamed the replicas
+    hbm_equiv = {"bytes_per_update": 8.0, "achieved_GBs": kernel_rate * 8.0 / 1e9,
+                 "peak": HBM_PEAK_GBS, "frac": kernel_rate * 8.0 / 1e9 / HBM_PEAK_GBS}
+
+    cpu, agreement, agreement_note = None, None, None
+    if not args.no_cpu_baseline:
+        plan = cpu_sample_plan()
+        cpu = cpu_baseline(plan)
+        cpu_bins = cpu.pop("bins")
+        # the FP64 comparison: the same realizations at T = 1.30 on the GPU,
+        # with the CPU sample's thermalization, measured sweeps and bins (a
+        # replica's chain does not depend on its batch, so these are the
+        # batch's replicas 8 r + 4)
+        therm, measured, cbin = plan
+        sub = [reps[8 * r + 4] for r in range(min(len(cpu_bins), args.realizations))]
+        g = slsf.Batch(sub, slsf.FP32)
+        g.init_angles()
+        g.run(therm, measured, cbin)
+        gpu_bins = {r: g.all_bins(r) for r in range(len(sub))}
+        g.close()
+        agreement = fp64_agreement(slsf, gpu_bins, cpu_bins, V, 1.0 / 1.3)
+        agreement_note = (f"separate sub-run: realizations 0..{len(sub) - 1} at T=1.30 on the "
+                          f"GPU (FP32) and on the CPU (FP64 reference mode), {therm} "
+                          f"thermalization + {measured} measured sweeps each, bins of {cbin}")
+    line(value, "site-updates/s", args.steps, args.warmup, 1e3 * wall / args.steps,
          {"workload": f"L=32 batched disorder average: {args.realizations} realizations x "
-                      "8 T in [1.10,1.45], Metropolis + 1 Wolff + 13-series measurement "
-                      "per sweep, 2000 thermalization + 10000 measured sweeps",
-          "replicas": len(reps), "L": 32, "binsize": args.binsize},
+                      f"8 T in [1.10,1.45], Metropolis + 1 Wolff + 13-series measurement "
+                      f"per sweep, {args.warmup} warmup (thermalization) + {args.steps} "
+                      f"timed measured sweeps, bins of {binsize}",
+          "replicas": len(reps), "L": 32, "binsize": binsize},
          "fp32 (fp64 accumulators)",
          plain_single_replica=plain,
          batched_over_plain=(value / plain) if plain else None,
          metropolis_only_rate=metro_rate, wolff_only_rate=wolff_rate,
          acceptance=st["acceptance_mean"], mean_cluster_size=st["mean_cluster_size"],
-         roofline=roofline, lds_roofline=lds, valu_issue=valu, ceilings=ceil,
+         roofline=roofline, valu_issue=valu, hbm_equivalent=hbm_equiv, ceilings=ceil,
          cpu_baseline=cpu, fp64_cpu_agreement=agreement,
+         fp64_cpu_agreement_run=agreement_note,
          kernel={"name": "k_resident<32, 1, paired>", "launches": st["launches"],
                  "avg_launch_ms": avg_launch_ms, "sweeps_per_launch": spl,
-                 "algorithmic_bytes_per_launch": alg_bytes_launch})
+                 "site_updates_per_launch": per_launch_updates,
+                 "rate_in_kernel": kernel_rate})
 
 
 def bench_c1(args, slsf):

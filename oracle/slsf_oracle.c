@@ -180,6 +180,45 @@ double or_exp(double x) {
 /* reference-mode math: libm stands in for sleef u10 (src/fast_math.rs:3) */
 static void ref_sincos(double x, double* s, double* c) { *s = sin(x); *c = cos(x); }
 
+/* Vector reference-mode math for the batch steps (src/fast_math.rs:5-25: sleef's
+ * sincos_u10/exp_u10 on 4 or 8 lanes).  Only the timed CPU-baseline builds define
+ * OR_VECTOR_MATH (8: AVX-512, 4: AVX2); they call glibc's libmvec, the system's
+ * vector libm (1-4 ulp, as sleef u10), through its x86_64 vector ABI names.  The
+ * strict checker build keeps the scalar libm calls above. */
+#if defined(OR_VECTOR_MATH)
+#include <immintrin.h>
+__m256d _ZGVdN4v_sin(__m256d);
+__m256d _ZGVdN4v_cos(__m256d);
+__m256d _ZGVdN4v_exp(__m256d);
+static void ref_sincos_n(int lanes, const double* x, double* s, double* c) {
+#if OR_VECTOR_MATH == 8
+  if (lanes == 8) {
+    extern __m512d _ZGVeN8v_sin(__m512d);
+    extern __m512d _ZGVeN8v_cos(__m512d);
+    const __m512d v = _mm512_loadu_pd(x);
+    _mm512_storeu_pd(s, _ZGVeN8v_sin(v));
+    _mm512_storeu_pd(c, _ZGVeN8v_cos(v));
+    return;
+  }
+#endif
+  for (int k = 0; k < lanes; k += 4) {
+    const __m256d v = _mm256_loadu_pd(x + k);
+    _mm256_storeu_pd(s + k, _ZGVdN4v_sin(v));
+    _mm256_storeu_pd(c + k, _ZGVdN4v_cos(v));
+  }
+}
+static void ref_exp_n(int lanes, const double* x, double* out) {
+#if OR_VECTOR_MATH == 8
+  if (lanes == 8) {
+    extern __m512d _ZGVeN8v_exp(__m512d);
+    _mm512_storeu_pd(out, _ZGVeN8v_exp(_mm512_loadu_pd(x)));
+    return;
+  }
+#endif
+  for (int k = 0; k < lanes; k += 4) _mm256_storeu_pd(out + k, _ZGVdN4v_exp(_mm256_loadu_pd(x + k)));
+}
+#endif
+
 /* ===================================================================== */
 /* Lattice, scratch -- src/types.rs:109-251,335-366                      */
 /* ===================================================================== */
@@ -310,7 +349,17 @@ void or_metropolis_accept_probabilities(const double* delta, int lanes, double b
     }
   }
   for (int lane = 0; lane < lanes; ++lane) prob[lane] = 1.0;
-  /* vector or scalar exp: libm for both here, so the branch is value-neutral */
+#if defined(OR_VECTOR_MATH)
+  /* vector exp of all lanes once enough of them are uphill (:42-51) */
+  if (lanes >= 4 && uphill >= or_vector_exp_min_uphill(lanes)) {
+    double e[8];
+    for (int k = uphill; k < lanes; ++k) arg[k] = 0.0;
+    ref_exp_n(lanes, arg, e);
+    for (int k = 0; k < uphill; ++k) prob[uphill_lane[k]] = e[k];
+    return;
+  }
+#endif
+  /* scalar exp (the checker build takes libm for both, value-neutral there) */
   for (int k = 0; k < uphill; ++k) prob[uphill_lane[k]] = exp(arg[k]);
 }
 
@@ -368,7 +417,11 @@ static int step_batch_ref(or_lattice* l, int lanes, double width, double beta, i
     local_field(l, idx, x, y, z, &hx[k], &hy[k]);
     oe[k] = local_energy(l->sin_t[idx], l->cos_t[idx], hx[k], hy[k]);
   }
+#if defined(OR_VECTOR_MATH)
+  ref_sincos_n(lanes, tn, ns, nc); /* vector sincos of the lanes (:338) */
+#else
   for (int k = 0; k < lanes; ++k) ref_sincos(tn[k], &ns[k], &nc[k]);
+#endif
   for (int k = 0; k < lanes; ++k) de[k] = local_energy(ns[k], nc[k], hx[k], hy[k]) - oe[k];
   or_metropolis_accept_probabilities(de, lanes, beta, prob);
   int accepted = 0;

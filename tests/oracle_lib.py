@@ -12,7 +12,22 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
-ORACLE_FAST_PATH = os.path.join(ROOT, "oracle", "liboracle_fast.so")
+ORACLE_FAST_PATHS = {8: os.path.join(ROOT, "oracle", "liboracle_fast_v4.so"),
+                     4: os.path.join(ROOT, "oracle", "liboracle_fast_v3.so")}
+
+
+def host_batch_lanes() -> int:
+    """8 on a host with AVX-512 (F and DQ), else 4: the reference's run-time
+    choice of its batch width (src/fast_math.rs:28-37, src/updates.rs:454)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return 8 if {"avx512f", "avx512dq"} <= flags else 4
+    except OSError:
+        pass
+    return 4
 
 _P = ctypes.POINTER
 _dp = _P(ctypes.c_double)
@@ -55,7 +70,9 @@ class OSimResult(ctypes.Structure):
 
 
 def load(fast: bool = False):
-    path = ORACLE_FAST_PATH if fast else ORACLE_PATH
+    """The strict checker library, or with fast=True the timed CPU-baseline build
+    for this host's batch width (host_batch_lanes)."""
+    path = ORACLE_FAST_PATHS[host_batch_lanes()] if fast else ORACLE_PATH
     if not os.path.exists(path):
         raise ImportError(f"{path} missing: run `make -C oracle`")
     L = ctypes.CDLL(path)
