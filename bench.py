@@ -298,6 +298,7 @@ def bench_c3(args, slsf):
     b = slsf.Batch(reps, slsf.FP32)
     b.init_angles()
     b.run(args.warmup, 0, 1, sweeps_per_launch=args.sweeps_per_launch)
+    b.reserve_bins(args.steps // binsize + 1)  # bin storage allocated before the clock starts
     # the timed region: exactly args.steps sweeps of every replica, with the
     # measurement of every sweep binned on the device; run() returns after the
     # batch's stream has drained, and the warmup run before it has too

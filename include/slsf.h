@@ -261,6 +261,12 @@ typedef struct {
 int slsf_autotune(slsf_batch* batch, int32_t trial_sweeps, slsf_tune_record* records,
                   int32_t cap, int32_t* count);
 
+/* Reserves device storage for nbins completed bins per (replica, series), so
+ * that measured runs up to that many bins allocate nothing (the analogue of
+ * reserving a Vec's capacity; no reference counterpart).  Completed bins are
+ * kept. */
+int slsf_reserve_bins(slsf_batch* batch, int32_t nbins);
+
 /* Completed bins of one series of one replica: writes min(cap, nbins) values
  * and returns the count in *nbins. */
 int slsf_get_bins(slsf_batch* batch, int32_t replica, int32_t series, double* out,

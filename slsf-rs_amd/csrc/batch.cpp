@@ -14,6 +14,7 @@
 #include "../../include/slsf.h"
 #include "common.h"
 #include "driver_session.h"
+#include "batch_state.h"
 #include "launch.h"
 #include "resident.h"
 
@@ -29,65 +30,9 @@ int fail(int code, const std::string& msg) {
 bool temperature_ok(double t) { return t > 0.0 && std::isfinite(t); }
 bool width_ok(double w) { return w >= 0.0 && std::isfinite(w); }
 
-#define HIP_TRY(expr)                                                                 \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return fail(SLSF_E_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + \
-                                     " at " #expr);                                   \
-  } while (0)
-
-struct HostReplica {
-  int lx, ly, lz, wolff_steps;
-  double temperature, width;
-  std::vector<double> j_xy, j_z;
-  uint64_t seed;
-  int64_t volume, padded, site_off;
-  int jz_off;
-};
-
 }  // namespace slsf
 
 using namespace slsf;
-
-struct slsf_batch {
-  int precision = SLSF_FP64_REPLAY;
-  hipStream_t stream = nullptr;
-  std::vector<HostReplica> reps;
-  std::vector<RepDev> reps_host;  // mirror of the device descriptors
-  GlobalState g;
-  int64_t total_layers = 0;
-  int64_t device_bytes = 0;
-  uint8_t* d_decisions = nullptr;
-  double* d_accum = nullptr;     // [nrep][NUM_SERIES]
-  double* d_bins = nullptr;      // [nrep][NUM_SERIES][bin_cap]
-  int32_t* d_bin_state = nullptr;  // [nrep][2]: measurements in open bin, completed bins
-  int32_t bin_cap = 0;
-  int32_t binsize = 0;
-  std::vector<double> acc_sum;   // per replica acceptance sums
-  std::vector<int64_t> acc_count;
-  std::vector<double> cluster_sum;
-  std::vector<int64_t> cluster_count;
-  std::vector<void*> allocations;
-  ResidentPlan resident;         // FP32 register/LDS-resident groups
-  bool initialised = false;
-
-  ~slsf_batch() {
-    resident_release(resident);
-    for (void* p : allocations) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-  template <typename T>
-  int alloc(T** p, size_t count) {
-    void* q = nullptr;
-    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-    HIP_TRY(hipMalloc(&q, bytes));
-    allocations.push_back(q);
-    device_bytes += (int64_t)bytes;
-    *p = (T*)q;
-    return SLSF_OK;
-  }
-};
 
 namespace {
 
@@ -193,16 +138,16 @@ int max_wolff_steps(const slsf_batch* b) {
   return m;
 }
 
-// per-replica sums of the TALLY_SLOTS slots of a device tally
+// per-replica sums of the TALLY_SLOTS slots of a device tally (summed on the
+// device; one small copy back)
 int read_tallies(slsf_batch* b, const unsigned long long* slots,
                  std::vector<unsigned long long>& out) {
-  std::vector<unsigned long long> all(b->reps.size() * TALLY_SLOTS);
-  HIP_TRY(hipMemcpyAsync(all.data(), slots, sizeof(unsigned long long) * all.size(),
+  out.assign(b->reps.size(), 0ull);
+  launch_sum_tallies(b->g, slots, b->d_tally_sum, b->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out.data(), b->d_tally_sum, sizeof(unsigned long long) * out.size(),
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  out.assign(b->reps.size(), 0ull);
-  for (size_t r = 0; r < out.size(); ++r)
-    for (int k = 0; k < TALLY_SLOTS; ++k) out[r] += all[r * TALLY_SLOTS + k];
   return SLSF_OK;
 }
 
@@ -321,6 +266,7 @@ int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t prec
   rc = rc ? rc : bp->alloc(&g.csize, (size_t)n * TALLY_SLOTS);
   rc = rc ? rc : bp->alloc(&g.acc, (size_t)n * TALLY_SLOTS);
   rc = rc ? rc : bp->alloc(&g.partials, (size_t)n * g.obs_blocks * 9);
+  rc = rc ? rc : bp->alloc(&bp->d_tally_sum, (size_t)n);
   if (rc) return rc;
   HIP_TRY(hipMemsetAsync(g.visited, 0, sizeof(uint32_t) * (size_t)(site_off / 32), b->stream));
   HIP_TRY(hipMemsetAsync(g.theta, 0, rs * (size_t)site_off, b->stream));
@@ -609,18 +555,50 @@ int slsf_get_acceptance(slsf_batch* b, double* acceptance) {
 // ---------------------------------------------------------------------------
 namespace {
 
+int ensure_accumulators(slsf_batch* b) {
+  const size_t nrep = b->reps.size();
+  if (b->d_accum) return SLSF_OK;
+  if (int rc = b->alloc(&b->d_accum, nrep * SLSF_NUM_SERIES)) return rc;
+  if (int rc = b->alloc(&b->d_bin_state, nrep * 2)) return rc;
+  HIP_TRY(hipMemsetAsync(b->d_accum, 0, sizeof(double) * nrep * SLSF_NUM_SERIES, b->stream));
+  HIP_TRY(hipMemsetAsync(b->d_bin_state, 0, sizeof(int32_t) * nrep * 2, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return SLSF_OK;
+}
+
+// room for `need` completed bins per (replica, series); a larger buffer takes
+// at least twice the old capacity, and the completed bins move over in one
+// pitched copy
+int grow_bins(slsf_batch* b, int64_t need) {
+  if (need <= b->bin_cap) return SLSF_OK;
+  if (need > INT32_MAX) return fail(SLSF_E_INVALID, "too many bins for one batch");
+  need = std::max<int64_t>(need, 2 * (int64_t)b->bin_cap);
+  need = std::min<int64_t>(need, INT32_MAX);
+  const size_t rows = b->reps.size() * SLSF_NUM_SERIES;
+  double* nb = nullptr;
+  if (int rc = b->alloc(&nb, rows * (size_t)need)) return rc;
+  if (b->d_bins) {  // keep completed bins
+    HIP_TRY(hipMemcpy2DAsync(nb, sizeof(double) * (size_t)need, b->d_bins,
+                             sizeof(double) * (size_t)b->bin_cap,
+                             sizeof(double) * (size_t)b->bin_cap, rows,
+                             hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->release(b->d_bins);
+    b->device_bytes -= (int64_t)(sizeof(double) * rows * (size_t)b->bin_cap);
+  }
+  b->d_bins = nb;
+  b->bin_cap = (int32_t)need;
+  return SLSF_OK;
+}
+
 int ensure_bins(slsf_batch* b, const slsf_run_options* opt) {
   const size_t nrep = b->reps.size();
-  if (!b->d_accum) {
-    if (int rc = b->alloc(&b->d_accum, nrep * SLSF_NUM_SERIES)) return rc;
-    if (int rc = b->alloc(&b->d_bin_state, nrep * 2)) return rc;
-    HIP_TRY(hipMemset(b->d_accum, 0, sizeof(double) * nrep * SLSF_NUM_SERIES));
-    HIP_TRY(hipMemset(b->d_bin_state, 0, sizeof(int32_t) * nrep * 2));
-  }
+  if (int rc = ensure_accumulators(b)) return rc;
   if (opt->sweeps == 0) return SLSF_OK;
   std::vector<int32_t> st(nrep * 2);
-  HIP_TRY(hipMemcpy(st.data(), b->d_bin_state, sizeof(int32_t) * st.size(),
-                    hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpyAsync(st.data(), b->d_bin_state, sizeof(int32_t) * st.size(),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   int32_t have = 0;
   bool measured = false;
   for (size_t r = 0; r < nrep; ++r) {
@@ -630,17 +608,7 @@ int ensure_bins(slsf_batch* b, const slsf_run_options* opt) {
   if (b->binsize != 0 && b->binsize != opt->binsize && measured)
     return fail(SLSF_E_STATE, "binsize must not change between measured runs of one batch");
   b->binsize = opt->binsize;
-  const int64_t need = (int64_t)have + opt->sweeps / opt->binsize + 1;
-  if (need <= b->bin_cap) return SLSF_OK;
-  double* nb = nullptr;
-  if (int rc = b->alloc(&nb, nrep * SLSF_NUM_SERIES * (size_t)need)) return rc;
-  if (b->d_bins)  // keep completed bins
-    for (size_t k = 0; k < nrep * SLSF_NUM_SERIES; ++k)
-      HIP_TRY(hipMemcpy(nb + k * need, b->d_bins + k * b->bin_cap, sizeof(double) * b->bin_cap,
-                        hipMemcpyDeviceToDevice));
-  b->d_bins = nb;
-  b->bin_cap = (int32_t)need;
-  return SLSF_OK;
+  return grow_bins(b, (int64_t)have + opt->sweeps / opt->binsize + 1);
 }
 
 }  // namespace
@@ -667,20 +635,17 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     return fail(SLSF_E_INVALID, "stream must be SLSF_STREAM_PRODUCTION or SLSF_STREAM_REPLAY");
   if (int rc = ensure_bins(b, opt)) return rc;
   // run window of every replica: measured from counter + thermalization, stop
-  // after thermalization + sweeps (kernels enforce both per replica)
-  if (int rc = sync_reps_from_device(b)) return rc;
-  for (auto& d : b->reps_host) {
-    d.measure_from = d.metro_ctr + (uint32_t)opt->thermalization;
-    d.stop_at = d.metro_ctr + (uint32_t)(opt->thermalization + opt->sweeps);
-    d.binsize = opt->binsize > 0 ? opt->binsize : 1;
-  }
-  if (int rc = push_reps_to_device(b)) return rc;
+  // after thermalization + sweeps (kernels enforce both per replica); set and
+  // closed again on the device, in stream order
+  launch_window(b->g, (uint32_t)opt->thermalization,
+                (uint32_t)(opt->thermalization + opt->sweeps),
+                opt->binsize > 0 ? opt->binsize : 1, 1, b->stream);
+  HIP_TRY(hipGetLastError());
   struct RestoreWindow {
     slsf_batch* b;
     ~RestoreWindow() {
-      if (sync_reps_from_device(b)) return;
-      for (auto& d : b->reps_host) { d.measure_from = UINT32_MAX; d.stop_at = UINT32_MAX; }
-      (void)push_reps_to_device(b);
+      launch_window(b->g, 0, 0, 1, 0, b->stream);
+      (void)hipStreamSynchronize(b->stream);
     }
   } restore{b};
   const auto t0 = std::chrono::steady_clock::now();
@@ -809,6 +774,14 @@ int slsf_autotune(slsf_batch* b, int32_t trial_sweeps, slsf_tune_record* records
   }
   if (count) *count = n;
   return SLSF_OK;
+}
+
+int slsf_reserve_bins(slsf_batch* b, int32_t nbins) {
+  clear_error();
+  if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
+  if (nbins < 0) return fail(SLSF_E_INVALID, "nbins must be nonnegative");
+  if (int rc = ensure_accumulators(b)) return rc;
+  return grow_bins(b, nbins);
 }
 
 int slsf_get_bins(slsf_batch* b, int32_t replica, int32_t series, double* out, int32_t cap,
@@ -957,11 +930,10 @@ int slsf_checkpoint_save(slsf_batch* b, void* buffer, int64_t bytes) {
     p += sizeof(double) * nrep * SLSF_NUM_SERIES;
     HIP_TRY(hipMemcpy(p, b->d_bin_state, sizeof(int32_t) * nrep * 2, hipMemcpyDeviceToHost));
     p += sizeof(int32_t) * nrep * 2;
-    for (size_t k = 0; k < nrep * SLSF_NUM_SERIES && h.nbins > 0; ++k) {
-      HIP_TRY(hipMemcpy(p, b->d_bins + k * b->bin_cap, sizeof(double) * h.nbins,
-                        hipMemcpyDeviceToHost));
-      p += sizeof(double) * h.nbins;
-    }
+    if (h.nbins > 0)
+      HIP_TRY(hipMemcpy2D(p, sizeof(double) * (size_t)h.nbins, b->d_bins,
+                          sizeof(double) * (size_t)b->bin_cap, sizeof(double) * (size_t)h.nbins,
+                          nrep * SLSF_NUM_SERIES, hipMemcpyDeviceToHost));
   }
   return SLSF_OK;
 }
@@ -1023,16 +995,12 @@ int slsf_checkpoint_load(slsf_batch* b, const void* buffer, int64_t bytes) {
     p += sizeof(double) * nrep * SLSF_NUM_SERIES;
     HIP_TRY(hipMemcpy(b->d_bin_state, p, sizeof(int32_t) * nrep * 2, hipMemcpyHostToDevice));
     p += sizeof(int32_t) * nrep * 2;
-    if (h.nbins > b->bin_cap) {
-      double* nb = nullptr;
-      if (int rc = b->alloc(&nb, nrep * SLSF_NUM_SERIES * (size_t)h.nbins)) return rc;
-      b->d_bins = nb;
-      b->bin_cap = h.nbins;
-    }
-    for (size_t k = 0; k < nrep * SLSF_NUM_SERIES && h.nbins > 0; ++k) {
-      HIP_TRY(hipMemcpy(b->d_bins + k * b->bin_cap, p, sizeof(double) * h.nbins,
-                        hipMemcpyHostToDevice));
-      p += sizeof(double) * h.nbins;
+    if (int rc = grow_bins(b, h.nbins)) return rc;
+    if (h.nbins > 0) {
+      HIP_TRY(hipMemcpy2D(b->d_bins, sizeof(double) * (size_t)b->bin_cap, p,
+                          sizeof(double) * (size_t)h.nbins, sizeof(double) * (size_t)h.nbins,
+                          nrep * SLSF_NUM_SERIES, hipMemcpyHostToDevice));
+      p += sizeof(double) * (size_t)h.nbins * nrep * SLSF_NUM_SERIES;
     }
   } else if (b->d_accum) {  // the image has no measurement: none is open afterwards
     HIP_TRY(hipMemset(b->d_accum, 0, sizeof(double) * nrep * SLSF_NUM_SERIES));

@@ -48,6 +48,45 @@ void set_extents(GlobalState& g, const RepDev* reps, int n) {
   }
 }
 
+// run window of every replica of a production run: measured from its counter
+// + thermalization, no sweep once the counter reaches + total; open = 0 closes
+// it again (nothing measured, nothing stopped).  Set on the device, so a run
+// needs no round trip of the descriptors.
+__global__ void k_window(RepDev* reps, int nrep, uint32_t therm, uint32_t total, int32_t binsize,
+                         int open) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  RepDev& d = reps[r];
+  if (open) {
+    d.measure_from = d.metro_ctr + therm;
+    d.stop_at = d.metro_ctr + total;
+    d.binsize = binsize;
+  } else {
+    d.measure_from = UINT32_MAX;
+    d.stop_at = UINT32_MAX;
+  }
+}
+
+// per-replica sums of the TALLY_SLOTS slots of a tally
+__global__ void k_sum_tallies(const unsigned long long* __restrict__ slots, int nrep,
+                              unsigned long long* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrep) return;
+  unsigned long long s = 0;
+  for (int k = 0; k < TALLY_SLOTS; ++k) s += slots[(int64_t)r * TALLY_SLOTS + k];
+  out[r] = s;
+}
+
+void launch_window(const GlobalState& g, uint32_t therm, uint32_t total, int32_t binsize,
+                   int open, hipStream_t s) {
+  k_window<<<(g.nrep + 63) / 64, 64, 0, s>>>(g.reps, g.nrep, therm, total, binsize, open);
+}
+
+void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
+                        unsigned long long* out, hipStream_t s) {
+  k_sum_tallies<<<(g.nrep + 63) / 64, 64, 0, s>>>(slots, g.nrep, out);
+}
+
 void launch_clear_counters(const GlobalState& g, hipStream_t s) {
   const int64_t n = (int64_t)g.nrep * TALLY_SLOTS;
   k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g.acc, n);

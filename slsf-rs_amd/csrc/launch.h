@@ -58,6 +58,13 @@ template <typename R> void launch_corr(const GlobalState& g, int replica, int64_
                                        double* out_xy, double* out_z, hipStream_t s);
 // zeroes the accepted-move and cluster-site tallies
 void launch_clear_counters(const GlobalState& g, hipStream_t s);
+// opens (open = 1) or closes the run window of every replica (RepDev
+// measure_from / stop_at / binsize) on the device
+void launch_window(const GlobalState& g, uint32_t therm, uint32_t total, int32_t binsize,
+                   int open, hipStream_t s);
+// out[r] = sum of replica r's TALLY_SLOTS slots
+void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
+                        unsigned long long* out, hipStream_t s);
 // folds a measured sweep into the bins; with rounds >= 0 also advances the
 // counters as launch_sweep_end(g, 1, rounds) would
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,

@@ -172,6 +172,7 @@ _SIGS = {
     "slsf_get_bins": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _dp,
                                      ctypes.c_int32, _i32p]),
     "slsf_get_acceptance": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "slsf_reserve_bins": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "slsf_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _P(TuneRecord),
                                      ctypes.c_int32, _i32p]),
     "slsf_binned_estimate": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_int32, _dp, _dp,
@@ -395,6 +396,11 @@ class Batch:
         _check(_lib.slsf_get_bins(self._h, replica, s, _ptr(out, ctypes.c_double), n.value,
                                   ctypes.byref(n)))
         return out[:n.value]
+
+    def reserve_bins(self, nbins: int):
+        """Device storage for nbins completed bins per replica and series, so
+        that measured runs up to that many bins allocate nothing."""
+        _check(_lib.slsf_reserve_bins(self._h, nbins))
 
     def all_bins(self, replica: int) -> np.ndarray:
         rows = [self.bins(replica, s) for s in range(len(SERIES))]
