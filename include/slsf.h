@@ -151,6 +151,22 @@ int slsf_measure_correlations(slsf_batch* batch, int32_t replica, int32_t rmax,
                               double* corr_y, double* corr_xy, double* corr_xy_by_z,
                               double* corr_z);
 
+/* slsf_measure_correlations without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, same values bit for bit; the row sums and the fold live in
+ * one fixed 16 MiB device array per GPU (2^21 doubles) that the library holds
+ * until the process ends.  A replica needs rxy*ly*lz*3 + rz*ly*lz + 3*rxy +
+ * lz*rxy + rz doubles of it (L = 64 cubic at the default radii: 0.53 M); larger
+ * requests fail with SLSF_E_INVALID and a message.  Every call on a GPU uses
+ * that GPU's one array: the library serialises its own calls per device until
+ * their work has finished, so calls from several host threads never overlap
+ * on it. */
+int slsf_measure_correlations_noalloc(slsf_batch* batch, int32_t replica, int32_t rmax,
+                                      int32_t rmax_xy, int32_t rmax_z, double* corr_x,
+                                      double* corr_y, double* corr_xy,
+                                      double* corr_xy_by_z, double* corr_z);
+
 /* ======================================================================== */
 /* Plain driver                                                              */
 /* ======================================================================== */

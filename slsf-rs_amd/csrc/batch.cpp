@@ -206,6 +206,7 @@ int slsf_batch_create(const slsf_replica_desc* replicas, int32_t n, int32_t prec
 
   auto b = std::make_unique<slsf_batch>();
   b->precision = precision;
+  HIP_TRY(hipGetDevice(&b->device));
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   int64_t site_off = 0;
   int jz_off = 0;

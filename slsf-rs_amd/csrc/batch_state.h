@@ -38,6 +38,7 @@ using namespace slsf;
 
 struct slsf_batch {
   int precision = SLSF_FP64_REPLAY;
+  int device = 0;                 // the HIP device the batch's memory lives on
   hipStream_t stream = nullptr;
   std::vector<HostReplica> reps;
   std::vector<RepDev> reps_host;  // mirror of the device descriptors
@@ -83,3 +84,5 @@ struct slsf_batch {
     return SLSF_OK;
   }
 };
+
+inline bool is_fp64_batch(const slsf_batch* b) { return b->precision == SLSF_FP64_REPLAY; }

@@ -167,6 +167,10 @@ _SIGS = {
     "slsf_measure_correlations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
                                                  ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.c_int32, _dp, _dp, _dp, _dp, _dp]),
+    "slsf_measure_correlations_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                         ctypes.c_int32, ctypes.c_int32,
+                                                         ctypes.c_int32, _dp, _dp, _dp, _dp,
+                                                         _dp]),
     "slsf_series_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "slsf_run": (ctypes.c_int, [ctypes.c_void_p, _P(RunOptions), _P(RunStats)]),
     "slsf_get_bins": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _dp,
@@ -367,13 +371,18 @@ class Batch:
                                           ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def correlations(self, replica=0, rmax=-1, rmax_xy=-1, rmax_z=-1) -> dict:
+    def correlations(self, replica=0, rmax=-1, rmax_xy=-1, rmax_z=-1,
+                     noalloc: bool = False) -> dict:
+        """C(r) of one replica (measure_theta_correlations_with_scratch); with
+        noalloc=True through the entry point that allocates no device memory."""
         rxy, rz = self.correlation_rmax(replica, rmax, rmax_xy, rmax_z)
         lz = self.replicas[replica].l_z
         cx, cy, cxy = (np.zeros(max(rxy, 1)) for _ in range(3))
         cbz = np.zeros(max(lz * rxy, 1))
         cz = np.zeros(max(rz, 1))
-        _check(_lib.slsf_measure_correlations(
+        fn = (_lib.slsf_measure_correlations_noalloc if noalloc
+              else _lib.slsf_measure_correlations)
+        _check(fn(
             self._h, replica, rmax, rmax_xy, rmax_z, *(_ptr(a, ctypes.c_double)
                                                        for a in (cx, cy, cxy, cbz, cz))))
         return {"corr_x": cx[:rxy], "corr_y": cy[:rxy], "corr_xy": cxy[:rxy],
