@@ -328,3 +328,85 @@ def test_driver_disorder_messages():  # src/tests.rs:354-369
     with pytest.raises(slsf.SlsfError) as e:
         slsf.run_theta_simulation(2, 2, 2, 1.0, 1.0, 0.0, 0.0)
     assert e.value.message == "temperature T must be positive and finite"
+
+
+def _synthetic_bins(rng, nb):
+    """Bins of the 13 series with realistic means and correlated fluctuations:
+    E and E^2, m^2 and m^4, and the helicity sums drawn jointly (the nonlinear
+    evaluables mix them)."""
+    e = -2.0 + 0.02 * rng.standard_normal(nb)
+    m2 = 0.6 + 0.05 * rng.standard_normal(nb)
+    m4 = m2 ** 2 + 0.01 + 0.01 * rng.standard_normal(nb)
+    cos = 3000.0 + 20.0 * rng.standard_normal((3, nb))
+    sin = 5.0 * rng.standard_normal((3, nb))
+    sin2 = sin ** 2 + 400.0 + 30.0 * rng.standard_normal((3, nb))
+    return np.vstack([e, e * e + 1e-4 * rng.standard_normal(nb), m2, m4, cos, sin, sin2])
+
+
+def _evaluables_numpy(mean, V, beta):
+    """The evaluables of src/model.rs:857-888 (plus the Binder ratio) from
+    series means, for the error checks below."""
+    rho = [mean[4 + a] / V - beta * (mean[10 + a] - mean[7 + a] ** 2) / V for a in range(3)]
+    return {"Magnetization": math.sqrt(mean[2]), "Chi": beta * V * mean[2],
+            "SpecificHeat": beta ** 2 * V * (mean[1] - mean[0] ** 2),
+            "RhoXY": (rho[0] + rho[1]) / 2, "RhoZ": rho[2], "Binder": mean[3] / mean[2] ** 2,
+            "Energy": mean[0]}
+
+
+def test_jackknife_errors_of_nonlinear_evaluables_equal_leave_one_out():
+    """slsf_evaluate's errors against a direct leave-one-out (each bin removed
+    with np.delete, the evaluable recomputed, sqrt((n-1)/n sum (f_i - f.)^2))."""
+    rng = np.random.default_rng(11)
+    V, beta = 32768.0, 1 / 1.3
+    for nb in (2, 3, 7, 40):
+        bins = _synthetic_bins(rng, nb)
+        ev = slsf.evaluate(bins, V, beta)
+        loo = [_evaluables_numpy(np.delete(bins, i, axis=1).mean(axis=1), V, beta)
+               for i in range(nb)]
+        for name in slsf.EVALS:
+            f = np.array([x[name] for x in loo])
+            err = math.sqrt((nb - 1) / nb * np.sum((f - f.mean()) ** 2))
+            assert abs(ev[name][1] - err) <= 1e-9 * max(abs(err), 1e-300), (nb, name)
+
+
+def test_jackknife_errors_approach_the_delta_method():
+    """At many bins the jackknife error of a smooth f equals the delta-method
+    error sqrt(g^T C g / n), g the analytic gradient of f at the means and C the
+    bins' covariance (an independent route to the same number, O(1/n) apart)."""
+    rng = np.random.default_rng(5)
+    V, beta, nb = 32768.0, 1 / 1.3, 4000
+    bins = _synthetic_bins(rng, nb)
+    ev = slsf.evaluate(bins, V, beta)
+    mu = bins.mean(axis=1)
+    C = np.cov(bins)
+    g = {k: np.zeros(13) for k in ("SpecificHeat", "RhoXY", "RhoZ", "Binder", "Magnetization")}
+    g["SpecificHeat"][0] = -2 * beta ** 2 * V * mu[0]
+    g["SpecificHeat"][1] = beta ** 2 * V
+    for a, w, name in ((0, 0.5, "RhoXY"), (1, 0.5, "RhoXY"), (2, 1.0, "RhoZ")):
+        g[name][4 + a] += w / V
+        g[name][10 + a] += -w * beta / V
+        g[name][7 + a] += w * 2 * beta * mu[7 + a] / V
+    g["Binder"][3] = 1 / mu[2] ** 2
+    g["Binder"][2] = -2 * mu[3] / mu[2] ** 3
+    g["Magnetization"][2] = 0.5 / math.sqrt(mu[2])
+    for name, grad in g.items():
+        delta = math.sqrt(grad @ C @ grad / nb)
+        assert abs(ev[name][1] / delta - 1) < 0.01, (name, ev[name][1], delta)
+
+
+def test_jackknife_error_matches_the_spread_over_independent_datasets():
+    """The reported error of the Binder ratio and the specific heat against the
+    actual standard deviation of the estimate over 400 independent synthetic
+    runs of 60 bins each (within 10%)."""
+    rng = np.random.default_rng(8)
+    V, beta = 32768.0, 1 / 1.3
+    vals = {"Binder": [], "SpecificHeat": [], "RhoZ": []}
+    errs = {k: [] for k in vals}
+    for _ in range(400):
+        ev = slsf.evaluate(_synthetic_bins(rng, 60), V, beta)
+        for k in vals:
+            vals[k].append(ev[k][0])
+            errs[k].append(ev[k][1])
+    for k in vals:
+        spread = np.std(vals[k], ddof=1)
+        assert abs(np.mean(errs[k]) / spread - 1) < 0.1, (k, np.mean(errs[k]), spread)

@@ -526,3 +526,80 @@ def test_binned_estimate_and_blocking():
     se1 = h.std(ddof=1) / math.sqrt(8)
     assert abs(L.or_blocking_stderr(y.ctypes.data_as(orc._dp), 16) - max(se0, se1)) <= 1e-12
     assert L.or_blocking_stderr(y.ctypes.data_as(orc._dp), 1) == 0.0
+
+
+def _corr_numpy(theta, lx, ly, lz, rxy, rz):
+    """C(r) of src/observables.rs:166-265 from the angles alone, by np.roll of
+    the (z, y, x) array: cos(theta_i - theta_{i + r e_a}) averaged over sites
+    (per plane for corr_xy_by_z), periodic in every direction."""
+    t = theta.reshape(lz, ly, lx)
+    cx = [np.cos(t - np.roll(t, -r, axis=2)) for r in range(1, rxy + 1)]
+    cy = [np.cos(t - np.roll(t, -r, axis=1)) for r in range(1, rxy + 1)]
+    cz = [np.cos(t - np.roll(t, -r, axis=0)) for r in range(1, rz + 1)]
+    corr_x = np.array([c.mean() for c in cx])
+    corr_y = np.array([c.mean() for c in cy])
+    by_z = np.array([[(cx[r][z].sum() + cy[r][z].sum()) / (2.0 * lx * ly)
+                      for r in range(rxy)] for z in range(lz)]).reshape(lz, rxy)
+    return {"corr_x": corr_x, "corr_y": corr_y, "corr_xy": (corr_x + corr_y) / 2,
+            "corr_xy_by_z": by_z, "corr_z": np.array([c.mean() for c in cz])}
+
+
+@pytest.mark.parametrize("shape,radii", [
+    ((6, 4, 8), (-1, -1, -1)),
+    ((10, 6, 4), (-1, -1, -1)),
+    ((8, 8, 12), (-1, -1, -1)),
+    ((4, 12, 6), (-1, -1, -1)),
+    ((12, 10, 8), (-1, 3, 2)),
+    ((16, 16, 16), (5, -1, -1)),
+    ((8, 8, 8), (-1, 0, 4)),
+])
+def test_correlations_equal_numpy_roll(shape, radii):
+    """Every output of or_measure_correlations on random, non-cubic lattices for
+    all r up to L/2, against an independent computation from the angles (an
+    x shift taken along y, or a wrong wrap of the second x run for r >= 2,
+    fails here)."""
+    lx, ly, lz = shape
+    lat = orc.Lattice(lx, ly, lz)
+    lat.set_couplings(np.ones(lz), np.ones(lz))
+    lat.init_keyed(1000 + lx * 31 + ly * 7 + lz)
+    lat.refresh(False)
+    c = lat.correlations(*radii)
+    rxy, rz = c["corr_x"].shape[0], c["corr_z"].shape[0]
+    rmax, rmax_xy, rmax_z = radii
+    exp_xy = min(rmax_xy if rmax_xy >= 0 else (rmax if rmax >= 0 else min(lx, ly) // 2),
+                 lx // 2, ly // 2)
+    exp_z = min(rmax_z if rmax_z >= 0 else (rmax if rmax >= 0 else lz // 2), lz // 2)
+    assert (rxy, rz) == (exp_xy, exp_z)
+    ref = _corr_numpy(lat.theta, lx, ly, lz, rxy, rz)
+    for k in ref:
+        assert c[k].shape == ref[k].shape, k
+        assert np.allclose(c[k], ref[k], rtol=0, atol=1e-12), (k, np.max(np.abs(c[k] - ref[k])))
+    # the random lattice leaves no two shifts alike, so a swapped axis shows
+    if rxy >= 1 and rz >= 1:
+        assert abs(ref["corr_x"][0] - ref["corr_y"][0]) > 1e-6
+        assert abs(ref["corr_x"][0] - ref["corr_z"][0]) > 1e-6
+
+
+@pytest.mark.parametrize("shape,k", [((8, 6, 10), (1, 2, 3)), ((12, 12, 4), (5, 0, 1)),
+                                     ((16, 8, 8), (3, 1, 2))])
+def test_correlations_of_a_spin_wave_closed_form(shape, k):
+    """theta = 2 pi (kx x / Lx + ky y / Ly + kz z / Lz): every site sees the
+    same neighbour angle difference, so C_a(r) = cos(2 pi k_a r / L_a) exactly
+    and every plane has corr_xy_by_z = (C_x + C_y) / 2."""
+    lx, ly, lz = shape
+    lat = orc.Lattice(lx, ly, lz)
+    lat.set_couplings(np.ones(lz), np.ones(lz))
+    z, y, x = np.meshgrid(np.arange(lz), np.arange(ly), np.arange(lx), indexing="ij")
+    th = np.mod(2 * np.pi * (k[0] * x / lx + k[1] * y / ly + k[2] * z / lz), 2 * np.pi)
+    lat.theta[:] = th.ravel()
+    lat.refresh(False)
+    c = lat.correlations()
+    rxy, rz = c["corr_x"].shape[0], c["corr_z"].shape[0]
+    r = np.arange(1, rxy + 1)
+    cx, cy = np.cos(2 * np.pi * k[0] * r / lx), np.cos(2 * np.pi * k[1] * r / ly)
+    assert np.allclose(c["corr_x"], cx, atol=1e-12)
+    assert np.allclose(c["corr_y"], cy, atol=1e-12)
+    assert np.allclose(c["corr_xy"], (cx + cy) / 2, atol=1e-12)
+    assert np.allclose(c["corr_xy_by_z"], np.tile((cx + cy) / 2, (lz, 1)), atol=1e-12)
+    rzv = np.arange(1, rz + 1)
+    assert np.allclose(c["corr_z"], np.cos(2 * np.pi * k[2] * rzv / lz), atol=1e-12)
