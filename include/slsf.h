@@ -260,11 +260,13 @@ typedef struct {
 int slsf_run(slsf_batch* batch, const slsf_run_options* opt, slsf_run_stats* stats);
 
 /* Autotuner of the FP32 batched kernel.  For every replica size of the batch
- * it times each candidate (z-columns per thread, i.e. waves per replica, and
- * sweeps per launch) on the batch's own replicas, keeps the fastest, and
- * restores the replicas' state and counters, so tuning does not change any
- * chain.  records (may be NULL) receives up to cap trial records; *count the
- * number of trials. */
+ * it times each candidate number of z-columns per thread (i.e. waves per
+ * replica) on the batch's own replicas, trial_sweeps sweeps in one launch per
+ * candidate, twice in alternation, keeps the fastest, and restores the
+ * replicas' state and counters, so tuning does not change any chain.  Runs keep
+ * the default launch length (the longest, which only gains; sweeps_per_launch
+ * of the records).  records (may be NULL) receives up to cap records, one per
+ * (size, candidate); *count the number of records. */
 typedef struct {
   int32_t l;                  /* replica edge of the group */
   int32_t columns_per_thread; /* 1 or 2 */

@@ -769,7 +769,7 @@ int slsf_autotune(slsf_batch* b, int32_t trial_sweeps, slsf_tune_record* records
       o.chosen = 0;
       o.reserved = 0;
       for (const auto& grp : b->resident.groups)
-        if (grp.L == r.L && grp.cpt == r.cpt && grp.sweeps_per_launch == r.spl) o.chosen = 1;
+        if (grp.L == r.L && grp.cpt == r.cpt) o.chosen = 1;
     }
     ++n;
   }

@@ -990,9 +990,15 @@ void resident_release(ResidentPlan& plan) {
   plan = ResidentPlan();
 }
 
-// Times every (columns per thread, sweeps per launch) candidate of each group
-// on the batch's own replicas without measurement, then restores the replicas'
-// angles, trig caches and counters, so tuning leaves no trace in the chains.
+// Times every columns-per-thread candidate of each group (waves per replica
+// L/4 or L/8 at L = 16, 24, 32) on the batch's own replicas without
+// measurement, then restores the replicas' angles, trig caches and counters, so
+// tuning leaves no trace in the chains.  Every candidate runs the same sweeps in
+// launches of the same length, twice in alternation, and keeps its faster time,
+// so neither launch overhead nor the order of the trials favours one of them.
+// The launch length stays the library's default: longer launches only gain
+// (fewer loads and stores of the replicas, fewer tails of idle compute units),
+// and the default is already the longest.
 int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
                       int trial_sweeps, std::vector<TuneRecord>& records) {
   records.clear();
@@ -1027,22 +1033,21 @@ int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
   }
+  const int sweeps = std::max(trial_sweeps, 1);
   for (size_t gi = 0; !rc && gi < plan.groups.size(); ++gi) {
     ResidentGroup& grp = plan.groups[gi];
-    TuneRecord best;
-    best.L = grp.L;
-    best.rate = -1.0;
-    for (int cpt : resident_cpt_candidates(grp.L))
-      for (int spl : {16, 64}) {
+    const std::vector<int> cands = resident_cpt_candidates(grp.L);
+    std::vector<float> best_ms(cands.size(), 0.f);
+    for (int pass = 0; !rc && pass < 2; ++pass)
+      for (size_t k = 0; !rc && k < cands.size(); ++k) {
         ResidentGroup trial = grp;
-        trial.cpt = cpt;
-        trial.sweeps_per_launch = spl;
+        trial.cpt = cands[k];
+        trial.sweeps_per_launch = sweeps;  // the trial is one launch
         ResidentRunArgs ra;
-        ra.thermalization = std::max(trial_sweeps, spl);
+        ra.thermalization = sweeps;
         ra.acc_out = acc;
         ra.csize_out = cs;
-        // warm the code object, then time
-        ResidentRunArgs warm = ra;
+        ResidentRunArgs warm = ra;  // loads the code object and warms the caches
         warm.thermalization = 1;
         enqueue_group(trial, g, warm, s);
         (void)hipEventRecord(e0, s);
@@ -1054,16 +1059,19 @@ int resident_autotune(ResidentPlan& plan, const GlobalState& g, hipStream_t s,
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
-        const double rate = (double)grp.replicas.size() * grp.L * grp.L * grp.L *
-                            (double)ra.thermalization / (ms * 1e-3);
-        TuneRecord rec{grp.L, cpt, spl, resident_waves_per_replica(grp.L, cpt), rate};
-        records.push_back(rec);
-        if (rate > best.rate) best = rec;
+        if (pass == 0 || ms < best_ms[k]) best_ms[k] = ms;
       }
-    if (!rc) {
-      grp.cpt = best.cpt;
-      grp.sweeps_per_launch = best.spl;
+    if (rc) break;
+    size_t best = 0;
+    for (size_t k = 0; k < cands.size(); ++k) {
+      const double rate = (double)grp.replicas.size() * grp.L * grp.L * grp.L * (double)sweeps /
+                          (best_ms[k] * 1e-3);
+      records.push_back(TuneRecord{grp.L, cands[k], default_sweeps_per_launch(grp.L),
+                                   resident_waves_per_replica(grp.L, cands[k]), rate});
+      if (best_ms[k] < best_ms[best]) best = k;
     }
+    grp.cpt = cands[best];
+    grp.sweeps_per_launch = 0;  // the default launch length
   }
   if (!rc && copy_all(false) != hipSuccess) rc = fail(SLSF_E_DEVICE, "autotune restore failed");
   if (e0) (void)hipEventDestroy(e0);

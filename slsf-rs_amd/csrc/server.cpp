@@ -177,6 +177,24 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   return SLSF_OK;
 }
 
+// Frees every pool other than size L that has no active slot and returns its
+// bytes to the budget, so that a job of a size without a pool is not kept out
+// by idle pools of other sizes.  A pool with no active slot has no work in
+// flight: its last chunk was waited for when its slots retired.
+void release_idle_pools(slsf_server* s, int L) {
+  for (auto it = s->pools.begin(); it != s->pools.end();) {
+    Pool& p = *it->second;
+    const bool idle = std::none_of(p.slot_job.begin(), p.slot_job.end(),
+                                   [](int64_t j) { return j >= 0; });
+    if (it->first != L && idle) {
+      s->used -= p.bytes;
+      it = s->pools.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
 int admit(slsf_server* s, Pool& p, int slot, Job& j) {
   const int L = p.L;
   const slsf_job_request& q = j.req;
@@ -404,6 +422,10 @@ int slsf_server_step(slsf_server* s, int32_t* finished) {
     if (pit == s->pools.end()) {
       std::unique_ptr<Pool> p;
       if (int rc = create_pool(s, it->req.l, p)) return rc;
+      if (!p) {  // no room: give back idle pools of other sizes and try again
+        release_idle_pools(s, it->req.l);
+        if (int rc = create_pool(s, it->req.l, p)) return rc;
+      }
       if (!p) { ++it; continue; }
       pit = s->pools.emplace(it->req.l, std::move(p)).first;
     }

@@ -33,12 +33,21 @@ def wrapped_diff(a, b):
     return np.minimum(d, 2 * math.pi - d)
 
 
-@pytest.mark.parametrize("L", [8, 16, 32, 48])
+# FP32 against FP64 on the same uniforms.  A decision can go the other way only
+# when a uniform lies within the FP32 error of its threshold (about 1e-7 of its
+# range), so at most one is allowed over a test's 0.004-0.9 M updates; an
+# agreeing site differs by the FP32 rounding of its angle in turns (measured
+# below 1e-6 rad).
+MAX_DECISION_MISMATCHES = 1
+MAX_ABS_DTHETA = 2e-6
+
+
+@pytest.mark.parametrize("L", [8, 12, 16, 24, 32, 48])
 def test_replay_stream_metropolis_decisions(L):
-    """Metropolis only (wolff_steps=0): decision mismatches per 1e6 updates
-    between the FP32 kernels and the FP64 replay kernel on the same uniforms
-    (L <= 32: the resident kernel's replay stream; L = 48: the FP32 HBM
-    kernels, which always draw the per-site stream)."""
+    """Metropolis only (wolff_steps=0): decision mismatches between the FP32
+    kernels and the FP64 replay kernel on the same uniforms (L <= 32: the
+    resident kernel's replay stream; L = 48: the FP32 HBM kernels, which always
+    draw the per-site stream)."""
     n = 8
     reps = reps_for(L, n, wolff=0)
     b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
@@ -56,16 +65,18 @@ def test_replay_stream_metropolis_decisions(L):
         if (~mism).any():
             maxd = max(maxd, float(d[~mism].max()))
     rate = bad / (n * L ** 3) * 1e6
-    print(f"L={L}: decision mismatches per 1e6 updates = {rate:.2f}, max |dtheta| = {maxd:.2e}")
-    assert rate < 200.0
-    assert maxd < 2e-5
+    print(f"L={L}: {bad} decision mismatches in {n * L ** 3} updates ({rate:.2f} per 1e6), "
+          f"max |dtheta| = {maxd:.2e}")
+    assert bad <= MAX_DECISION_MISMATCHES
+    assert maxd < MAX_ABS_DTHETA
 
 
-@pytest.mark.parametrize("L", [8, 16, 48])
+@pytest.mark.parametrize("L", [8, 12, 16, 24, 32, 48])
 def test_replay_stream_full_sweep(L):
-    """Metropolis + Wolff: most replicas reproduce the FP64 sweep to FP32
-    rounding (a cluster flips differently only if a bond's uniform falls within
-    FP32 error of its probability)."""
+    """Metropolis + Wolff on the same uniforms: every replica reproduces the
+    FP64 sweep to FP32 rounding (a cluster flips differently only if a bond's
+    uniform falls within FP32 error of its probability); at most one replica
+    may differ by a decision (one site, or one cluster)."""
     n = 16
     reps = reps_for(L, n, T=1.1)
     b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
@@ -74,11 +85,16 @@ def test_replay_stream_full_sweep(L):
     b32.init_angles()
     b64.sweep(1)
     b32.run(1, 0, 1, replay_stream=True)
-    same = 0
+    differ, maxd = 0, 0.0
     for r in range(n):
         d = wrapped_diff(b64.get_theta(r), b32.get_theta(r))
-        same += int(d.max() < 2e-5 or (d > 1e-3).mean() < 0.002)
-    assert same >= n - 2, same
+        if d.max() < MAX_ABS_DTHETA:
+            maxd = max(maxd, float(d.max()))
+        else:
+            differ += 1
+    print(f"L={L}: {differ} of {n} replicas differ by a decision, max |dtheta| of the "
+          f"others {maxd:.2e}")
+    assert differ <= 1, differ
     m64, w64 = b64.counters()
     m32, w32 = b32.counters()
     assert list(m64) == list(m32) and list(w64) == list(w32)
@@ -287,13 +303,39 @@ def test_autotune_picks_and_leaves_chains_unchanged():
     assert {r["l"] for r in recs} == {8, 12, 16, 24, 32}
     for L in (8, 12, 16, 24, 32):
         chosen = [r for r in recs if r["l"] == L and r["chosen"]]
-        assert len(chosen) >= 1 and all(r["site_updates_per_s"] > 0 for r in recs)
+        assert len(chosen) == 1 and all(r["site_updates_per_s"] > 0 for r in recs)
+    assert all(r["sweeps_per_launch"] == 1000 for r in recs)  # the default launch length
+    for L in (8, 12, 16, 24, 32):
+        best = max(r["site_updates_per_s"] for r in recs if r["l"] == L)
+        assert [r["site_updates_per_s"] for r in recs if r["l"] == L and r["chosen"]] == [best]
     assert list(a.counters()[0]) == [0] * len(reps)
     a.run(10, 10, 5)
     b.run(10, 10, 5)
     for i in range(len(reps)):
         assert np.array_equal(a.get_theta(i), b.get_theta(i)), i
         assert np.allclose(a.all_bins(i), b.all_bins(i), rtol=2e-5, atol=1e-6)
+
+
+def test_autotuned_batch_is_not_slower_than_the_default():
+    """The uneven batch at 16 replicas per size: a timed run after autotuning
+    against the same run with the default plan (same replicas, same sweeps);
+    the tuned run may not be slower beyond timing noise (3%)."""
+    reps = []
+    for i, L in enumerate([8, 12, 16, 24, 32]):
+        reps += reps_for(L, 16, T=1.1 + 0.1 * i, seed0=91 * L)
+    rates = {}
+    for tuned in (False, True, False, True):
+        b = slsf.Batch(reps, slsf.FP32)
+        b.init_angles()
+        if tuned:
+            b.autotune(32)
+        b.run(20, 0, 1)
+        st = b.run(0, 400, 20)
+        rate = sum(r.volume for r in reps) * 400 / st["seconds"]
+        rates[tuned] = max(rates.get(tuned, 0.0), rate)
+        b.close()
+    print(f"default {rates[False]:.3e}, autotuned {rates[True]:.3e} site-updates/s")
+    assert rates[True] >= 0.97 * rates[False]
 
 
 def test_graph_capture_matches_stream_launches():

@@ -99,6 +99,25 @@ def test_memory_budget_admission():
     assert e.value.message == "red-black Metropolis sweep requires even lattice dimensions"
 
 
+def test_idle_pools_of_other_sizes_give_their_memory_back():
+    """A budget with room for one L=16 slot or one L=32 slot, not both: the
+    L=16 job's pool takes the budget first; once that job is done, its idle
+    pool is freed so that the queued L=32 job is admitted, and both finish."""
+    b16, b32 = slsf.Server.slot_bytes(16), slsf.Server.slot_bytes(32)
+    budget = max(b16, b32) + min(b16, b32) // 2
+    srv = slsf.Server(budget, 8, chunk_sweeps=50)
+    a = srv.submit(**job_args(16, 11, sweeps=100))
+    b = srv.submit(**job_args(32, 12, sweeps=100))
+    srv.step()
+    assert srv.stats()["active"] == 1 and srv.stats()["queued"] == 1
+    res = {r["job_id"]: r for r in srv.run_until_idle(max_steps=1000)}
+    assert set(res) == {a, b}, res.keys()
+    assert res[b]["start_s"] >= res[a]["finish_s"] - 1e-9
+    st = srv.stats()
+    assert st["queued"] == 0 and st["active"] == 0 and st["device_bytes"] <= budget
+    srv.close()
+
+
 def test_served_job_equals_batch_run():
     """A served job (resident pool for L=16, HBM pool for L=48, chunks of 30
     sweeps) gives exactly the bins of the same replica run through Batch.run:
