@@ -95,6 +95,14 @@ int push_reps_to_device(slsf_batch* b) {
 
 bool is_fp64(const slsf_batch* b) { return b->precision == SLSF_FP64_REPLAY; }
 
+// the second angle buffer and tile partials of the slab sweeps, on first use
+int ensure_slab_buffers(slsf_batch* b) {
+  if (b->g.theta2) return SLSF_OK;
+  const size_t rs = is_fp64(b) ? sizeof(double) : sizeof(float);
+  if (int rc = b->alloc((char**)&b->g.theta2, rs * (size_t)b->g.total_sites)) return rc;
+  return b->alloc(&b->g.slab_partials, (size_t)b->g.nrep * b->g.max_tiles8 * 9);
+}
+
 int metro_phases(slsf_batch* b, uint8_t* d_decisions) {
   if (is_fp64(b)) launch_metro_phases<double>(b->g, d_decisions, b->stream);
   else launch_metro_phases<float>(b->g, d_decisions, b->stream);
@@ -186,6 +194,15 @@ std::vector<int64_t> clusters_in(const slsf_batch* b, int64_t nsweeps) {
 }
 
 }  // namespace
+
+int ensure_trig_cache(slsf_batch* b) {
+  if (!b->cs_stale) return SLSF_OK;
+  if (is_fp64(b)) launch_refresh<double>(b->g, b->stream);
+  else launch_refresh<float>(b->g, b->stream);
+  HIP_TRY(hipGetLastError());
+  b->cs_stale = false;
+  return SLSF_OK;
+}
 
 extern "C" {
 
@@ -304,6 +321,7 @@ int slsf_init_angles(slsf_batch* b, int32_t cold) {
   std::fill(b->cluster_sum.begin(), b->cluster_sum.end(), 0.0);
   std::fill(b->cluster_count.begin(), b->cluster_count.end(), 0);
   b->initialised = true;
+  b->cs_stale = false;
   return SLSF_OK;
 }
 
@@ -355,6 +373,7 @@ int slsf_set_theta(slsf_batch* b, int32_t replica, const double* theta) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->initialised = true;
+  b->cs_stale = false;
   return SLSF_OK;
 }
 
@@ -382,6 +401,7 @@ int slsf_metropolis_sweep(slsf_batch* b, double* acceptance, uint8_t* decisions)
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (int rc = ensure_trig_cache(b)) return rc;
   if (decisions && !b->d_decisions)
     if (int rc = b->alloc(&b->d_decisions, (size_t)b->g.total_sites)) return rc;
   if (acceptance) std::fill(acceptance, acceptance + b->reps.size(), 0.0);
@@ -408,6 +428,7 @@ int slsf_wolff_cluster_step(slsf_batch* b, int64_t* cluster_size, uint8_t* membe
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (int rc = ensure_trig_cache(b)) return rc;
   // a standalone cluster step runs one cluster for every replica (round 0 with
   // wolff_steps forced >= 1 would change the descriptor; use a dedicated path)
   if (int rc = sync_reps_from_device(b)) return rc;
@@ -437,6 +458,7 @@ int slsf_sweep(slsf_batch* b, int32_t nsweeps, double* acceptance_sum) {
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (int rc = ensure_trig_cache(b)) return rc;
   if (nsweeps < 0) return fail(SLSF_E_INVALID, "nsweeps must be nonnegative");
   const int wmax = max_wolff_steps(b);
   launch_clear_counters(b->g, b->stream);
@@ -450,6 +472,7 @@ int slsf_sweep(slsf_batch* b, int32_t nsweeps, double* acceptance_sum) {
 int slsf_measure_observables(slsf_batch* b, slsf_observables* out) {
   clear_error();
   if (!b || !out) return fail(SLSF_E_INVALID, "null argument");
+  if (int rc = ensure_trig_cache(b)) return rc;
   if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
   else launch_observables<float>(b->g, b->stream);
   HIP_TRY(hipGetLastError());
@@ -493,6 +516,7 @@ int slsf_measure_correlations(slsf_batch* b, int32_t replica, int32_t rmax, int3
                               double* corr_xy_by_z, double* corr_z) {
   int32_t rxy = 0, rz = 0;
   if (int rc = slsf_correlation_rmax(b, replica, rmax, rmax_xy, rmax_z, &rxy, &rz)) return rc;
+  if (int rc = ensure_trig_cache(b)) return rc;
   const auto& r = b->reps[replica];
   const int64_t rows = (int64_t)r.ly * r.lz;
   double* d_xy = nullptr;
@@ -682,6 +706,8 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     // replicas' windows cut thermalization (no measurement) and the tail, and
     // replicas the resident kernel already ran to their window end sit out
     const int wmax = max_wolff_steps(b);
+    if (int rc = ensure_slab_buffers(b)) return rc;
+    b->cs_stale = true;
     if (!resident_part) {
       launch_clear_counters(b->g, b->stream);
       launches += 2;
@@ -898,6 +924,7 @@ int slsf_checkpoint_save(slsf_batch* b, void* buffer, int64_t bytes) {
   size_t need = 0;
   if (int rc = ckpt_layout(b, h, need)) return rc;
   if (bytes < (int64_t)need) return fail(SLSF_E_INVALID, "checkpoint buffer is too small");
+  if (int rc = ensure_trig_cache(b)) return rc;
   if (int rc = sync_reps_from_device(b)) return rc;
   const size_t nrep = b->reps.size();
   char* p = static_cast<char*>(buffer);
@@ -1009,6 +1036,7 @@ int slsf_checkpoint_load(slsf_batch* b, const void* buffer, int64_t bytes) {
   }
   b->binsize = h.binsize;
   b->initialised = true;
+  b->cs_stale = false;  // the image holds the cache of its angles
   return SLSF_OK;
 }
 
@@ -1034,6 +1062,7 @@ struct DriverSession {
 int driver_open(slsf_batch* b, int64_t nmeas, int rxy, int rz, DriverSession** out) {
   if (!b || !out) return fail(SLSF_E_INVALID, "null argument");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
+  if (int rc = ensure_trig_cache(b)) return rc;
   auto s = std::make_unique<DriverSession>();
   s->b = b;
   s->nmeas = nmeas;

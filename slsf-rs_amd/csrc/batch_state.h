@@ -59,6 +59,7 @@ struct slsf_batch {
   std::vector<void*> allocations;
   ResidentPlan resident;         // FP32 register/LDS-resident groups
   bool initialised = false;
+  bool cs_stale = false;         // slab sweeps left the (cos, sin) cache behind theta
 
   ~slsf_batch() {
     resident_release(resident);
@@ -86,3 +87,7 @@ struct slsf_batch {
 };
 
 inline bool is_fp64_batch(const slsf_batch* b) { return b->precision == SLSF_FP64_REPLAY; }
+
+// Brings the (cos, sin) cache up to date after slab sweeps (recomputed from the
+// angles with the kernels' own sincos, which gives the cache's values).
+int ensure_trig_cache(slsf_batch* b);

@@ -4,6 +4,7 @@
 
 #include "launch.h"
 #include "global_kernels.inc"
+#include "slab.inc"
 #define SLSF_REAL float
 #include "kernels_body.inc"
 #undef SLSF_REAL
@@ -114,36 +115,9 @@ k_accumulate(RepDev* reps, const double* __restrict__ partials, int obs_stride, 
   RepDev& d = reps[r];
   const int lane = threadIdx.x;
   const bool measured = d.active && d.metro_ctr >= d.measure_from;
-  if (measured) {
-    const int nb = obs_blocks_of(d.lx, d.ly, d.lz);
-    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = lane; b < nb; b += 64)
-#pragma unroll
-      for (int k = 0; k < 9; ++k) v[k] += partials[((int64_t)r * obs_stride + b) * 9 + k];
-#pragma unroll
-    for (int k = 0; k < 9; ++k)
-      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-    if (lane == 0) {
-      const int32_t binsize = d.binsize;
-      const double vol = (double)d.volume;
-      const double e = v[0] / vol;
-      const double m2 = (v[1] * v[1] + v[2] * v[2]) / (vol * vol);
-      const double vals[13] = {e,    e * e, m2,   m2 * m2,     v[3],        v[4],       v[5],
-                               v[6], v[7],  v[8], v[6] * v[6], v[7] * v[7], v[8] * v[8]};
-      double* acc = accum + (int64_t)r * 13;
-      for (int k = 0; k < 13; ++k) acc[k] += vals[k];
-      int32_t* st = bin_state + 2 * r;
-      st[0] += 1;
-      if (st[0] == binsize) {
-        if (st[1] < bin_cap)
-          for (int k = 0; k < 13; ++k)
-            bins[((int64_t)r * 13 + k) * bin_cap + st[1]] = acc[k] / (double)binsize;
-        for (int k = 0; k < 13; ++k) acc[k] = 0.0;
-        st[0] = 0;
-        st[1] += 1;
-      }
-    }
-  }
+  if (measured)
+    fold_measurement(d, r, partials + (int64_t)r * obs_stride * 9,
+                     obs_blocks_of(d.lx, d.ly, d.lz), accum, bins, bin_state, bin_cap, lane);
   if (rounds >= 0 && lane == 0) {  // as k_sweep_end(metro = 1, rounds), after every read of d
     if (d.active) {
       d.metro_ctr += 1u;

@@ -2,6 +2,7 @@
 // evaluated as written, which the bit-exact replay parity relies on.
 #include "launch.h"
 #include "global_kernels.inc"
+#include "slab.inc"
 #include "replay_resident.inc"
 #define SLSF_REAL double
 #include "kernels_body.inc"

@@ -30,6 +30,10 @@ struct GlobalState {
   unsigned long long* acc = nullptr;  // [nrep * TALLY_SLOTS] accepted moves, summed until cleared
   double* partials = nullptr;         // [nrep * obs_blocks * 9]
   int obs_blocks = 0;                 // max over replicas of obs_blocks_of
+  // slab sweeps (enqueue_hbm_sweeps): the second angle buffer, R[total_sites],
+  // and the measurement partials of the tiles, [nrep * max_tiles8 * 9]
+  void* theta2 = nullptr;
+  double* slab_partials = nullptr;
 };
 
 // grid extents of a GlobalState from its replica descriptors (host side)
@@ -121,8 +125,12 @@ struct BinTarget {
   int32_t* bin_state = nullptr;
   int32_t bin_cap = 0;
 };
-// Enqueues `nsweeps` sweeps of the HBM path on s: Metropolis, `wmax` cluster
-// rounds, windowed observables + accumulation, counters.  Every sweep issues
+// Enqueues `nsweeps` slab sweeps of the HBM path on s (slab.inc): per sweep
+// k_slab (measurement of the incoming state, both Metropolis colours, the
+// first cluster round's bonds and tile components), k_wolff_hook and
+// k_slab_flip (reflection, bins, counters), then the extra cluster rounds; the
+// run ends with the closing measurement.  Needs g.theta2 and g.slab_partials,
+// and leaves the (cos, sin) cache stale (launch_refresh).  Every sweep issues
 // the same launches (the replicas' windows decide who takes part), so the
 // sequence can be captured into a graph and replayed.  Returns the launch count.
 template <typename R> int64_t enqueue_hbm_sweeps(const GlobalState& g, int64_t nsweeps, int wmax,

@@ -107,6 +107,7 @@ extern "C" int slsf_measure_correlations_noalloc(slsf_batch* b, int32_t replica,
                                                  double* corr_z) {
   int32_t rxy = 0, rz = 0;
   if (int rc = slsf_correlation_rmax(b, replica, rmax, rmax_xy, rmax_z, &rxy, &rz)) return rc;
+  if (int rc = ensure_trig_cache(b)) return rc;
   const auto& r = b->reps[replica];
   const int64_t rows = (int64_t)r.ly * r.lz;
   const int64_t n_xy = (int64_t)rxy * rows * 3, n_z = (int64_t)rz * rows;

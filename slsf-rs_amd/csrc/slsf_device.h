@@ -215,7 +215,7 @@ struct RepDev {
   uint32_t stop_at;       // the replica does no sweep once metro_ctr reaches this
   uint32_t active;        // HBM path: this replica takes part in the current sweep
   int32_t binsize;        // measurements per bin
-  int32_t pad0;
+  int32_t pending;        // slab path: the state after the last sweep is still to be measured
   double beta, width;
   uint64_t div_lx, div_hx, div_plane;  // divisor_magic of lx, lx/2, lx*ly
   uint64_t pad1[2];
@@ -258,6 +258,8 @@ struct WolffDev {  // per replica, per cluster step
   int64_t seed_site;
   uint32_t step;   // cluster counter used for the keyed uniforms
   int32_t active;  // 0 when this replica does no cluster in this round
+  int32_t swept;   // slab path: the replica's Metropolis sweep ran in this sweep's k_slab
+  int32_t pad;
 };
 
 }  // namespace slsf
