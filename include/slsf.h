@@ -239,6 +239,11 @@ typedef struct {
                                keyed Philox call per two site updates) or
                                SLSF_STREAM_REPLAY (the FP64 replay path's per-site
                                uniforms, for decision-mismatch statistics) */
+  int32_t hbm_form;         /* replicas on the HBM kernels: 0 = by launch size,
+                               1 = slab sweeps (three launches a sweep), 2 =
+                               per-phase sweeps (seven); the same chains bit for
+                               bit, the same bins up to FP64 summation order */
+  int32_t reserved;         /* 0 */
 } slsf_run_options;
 
 #define SLSF_STREAM_PRODUCTION 0

@@ -706,8 +706,12 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     // replicas' windows cut thermalization (no measurement) and the tail, and
     // replicas the resident kernel already ran to their window end sit out
     const int wmax = max_wolff_steps(b);
+    if (opt->hbm_form < 0 || opt->hbm_form > 2)
+      return fail(SLSF_E_INVALID, "hbm_form must be 0 (by launch size), 1 (slab) or 2 (per-phase)");
     if (int rc = ensure_slab_buffers(b)) return rc;
-    b->cs_stale = true;
+    const bool slab = hbm_uses_slab(b->g, opt->hbm_form);
+    if (slab) b->cs_stale = true;  // slab sweeps leave the trig cache behind
+    else if (int rc = ensure_trig_cache(b)) return rc;  // the per-phase kernels read it
     if (!resident_part) {
       launch_clear_counters(b->g, b->stream);
       launches += 2;
@@ -722,8 +726,8 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
       bt.bin_cap = b->bin_cap;
     }
     auto enqueue = [&](int64_t n) {
-      return is_fp64(b) ? enqueue_hbm_sweeps<double>(b->g, n, wmax, bt, b->stream)
-                        : enqueue_hbm_sweeps<float>(b->g, n, wmax, bt, b->stream);
+      return is_fp64(b) ? enqueue_hbm_sweeps<double>(b->g, n, wmax, bt, b->stream, opt->hbm_form)
+                        : enqueue_hbm_sweeps<float>(b->g, n, wmax, bt, b->stream, opt->hbm_form);
     };
     if (opt->use_graph && total > 0) {
       // one graph of `chunk` sweeps, replayed; sweeps past the window are no-ops

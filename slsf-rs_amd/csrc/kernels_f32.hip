@@ -88,6 +88,17 @@ void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
   k_sum_tallies<<<(g.nrep + 63) / 64, 64, 0, s>>>(slots, g.nrep, out);
 }
 
+// The slab form wins while a launch has at most two 16-deep tiles per compute
+// unit (measured per sweep, FP32: one L=96 replica 65 against 81 us, one L=128
+// replica 107 against 120 us, two L=96 replicas 97 against 99 us); beyond that
+// the per-phase kernels' small independent blocks fill the device better (four
+// L=128 replicas 254 against 315 us).
+bool hbm_uses_slab(const GlobalState& g, int form) {
+  if (form == 1) return true;
+  if (form == 2) return false;
+  return (int64_t)g.max_tiles * g.nrep <= 2 * WOLFF_SHALLOW_BELOW;
+}
+
 void launch_clear_counters(const GlobalState& g, hipStream_t s) {
   const int64_t n = (int64_t)g.nrep * TALLY_SLOTS;
   k_fill_u64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g.acc, n);

@@ -125,15 +125,28 @@ struct BinTarget {
   int32_t* bin_state = nullptr;
   int32_t bin_cap = 0;
 };
-// Enqueues `nsweeps` slab sweeps of the HBM path on s (slab.inc): per sweep
-// k_slab (measurement of the incoming state, both Metropolis colours, the
-// first cluster round's bonds and tile components), k_wolff_hook and
-// k_slab_flip (reflection, bins, counters), then the extra cluster rounds; the
-// run ends with the closing measurement.  Needs g.theta2 and g.slab_partials,
-// and leaves the (cos, sin) cache stale (launch_refresh).  Every sweep issues
-// the same launches (the replicas' windows decide who takes part), so the
+// Enqueues `nsweeps` sweeps of the HBM path on s in one of two forms that give
+// the same chains (bit for bit) and the same bins up to the order of their FP64
+// sums:
+//  * slab sweeps (slab.inc): per sweep k_slab (measurement of the incoming
+//    state, both Metropolis colours, the first cluster round's bonds and tile
+//    components), k_wolff_hook and k_slab_flip (reflection, bins, counters),
+//    then the extra cluster rounds; the run ends with the closing measurement.
+//    Three launches a sweep, which suits launches too small to fill the device
+//    (their time is the latency of the dependent launches).  Leaves the (cos,
+//    sin) cache stale (launch_refresh).
+//  * per-phase sweeps: two Metropolis colours, three cluster launches per
+//    round, the measurement and the counters (seven launches a sweep), whose
+//    small independent blocks fill the device better when the launch is large.
+// form 1 and 2 force the slab and the per-phase form; 0 chooses by the launch
+// size (hbm_uses_slab).  Both need g.theta2 and g.slab_partials.  Every sweep
+// issues the same launches (the replicas' windows decide who takes part), so the
 // sequence can be captured into a graph and replayed.  Returns the launch count.
 template <typename R> int64_t enqueue_hbm_sweeps(const GlobalState& g, int64_t nsweeps, int wmax,
-                                                 const BinTarget& bt, hipStream_t s);
+                                                 const BinTarget& bt, hipStream_t s,
+                                                 int form = 0);
+// whether enqueue_hbm_sweeps takes the slab form: forced (form 1 / 2) or, for
+// form 0, when the launch has at most two 16-deep tiles per compute unit
+bool hbm_uses_slab(const GlobalState& g, int form);
 
 }  // namespace slsf

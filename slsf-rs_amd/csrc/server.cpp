@@ -93,6 +93,7 @@ struct Pool {
   std::vector<int32_t> active_idx;
   ResidentGroup group;            // active slots of a resident pool
   bool running = false;
+  bool cs_stale = false;          // slab chunks left the trig cache behind the angles
 
   ~Pool() {
     for (void* p : allocs) (void)hipFree(p);
@@ -278,6 +279,10 @@ int run_chunk(slsf_server* s, Pool& p) {
     for (int i : p.active_idx) wmax = std::max(wmax, p.host[i].wolff_steps);
     GlobalState g = p.g;  // launch only up to the highest active slot
     g.nrep = p.active_idx.back() + 1;
+    // the per-phase form reads the trig cache that slab chunks leave behind
+    const bool slab = hbm_uses_slab(g, 0);
+    if (!slab && p.cs_stale) launch_refresh<float>(p.g, p.stream);
+    p.cs_stale = slab;
     launch_sweep_end(g, 0, 0, p.stream);  // marks the slots whose window is open
     BinTarget bt;
     bt.accum = p.accum;

@@ -60,7 +60,8 @@ class Observables(ctypes.Structure):
 class RunOptions(ctypes.Structure):
     _fields_ = [("thermalization", ctypes.c_int64), ("sweeps", ctypes.c_int64),
                 ("binsize", ctypes.c_int32), ("sweeps_per_launch", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("stream", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("stream", ctypes.c_int32),
+                ("hbm_form", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class RunStats(ctypes.Structure):
@@ -390,9 +391,12 @@ class Batch:
 
     def run(self, thermalization: int, sweeps: int, binsize: int,
             sweeps_per_launch: int = 0, use_graph: bool = False,
-            replay_stream: bool = False) -> dict:
+            replay_stream: bool = False, hbm_form: int = 0) -> dict:
+        """slsf_run: thermalization + measured sweeps with on-device bins.
+        hbm_form (replicas on the HBM kernels): 0 by launch size, 1 slab
+        sweeps, 2 per-phase sweeps -- the same chains and bins either way."""
         opt = RunOptions(thermalization, sweeps, binsize, sweeps_per_launch, int(use_graph),
-                         int(replay_stream))
+                         int(replay_stream), int(hbm_form), 0)
         st = RunStats()
         _check(_lib.slsf_run(self._h, ctypes.byref(opt), ctypes.byref(st)))
         return {n: getattr(st, n) for n, _ in RunStats._fields_}

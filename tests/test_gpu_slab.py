@@ -145,3 +145,51 @@ def test_slab_fp32_follows_fp64_on_the_same_uniforms():
         d = np.minimum(d, 2 * np.pi - d)
         differ += int(d.max() >= 2e-6)
     assert differ <= 1, differ
+
+
+def test_slab_replica_alone_and_in_a_large_batch_bit_identical():
+    """Slab sweeps of a replica alone take 8-deep tiles, of the same replica in
+    a batch of 256 or more 16-deep tiles 16-deep ones: its FP32 chain and its
+    bins are bit-identical either way (the measurement is summed per 8-deep half
+    tile in both).  With the form chosen by launch size (per-phase sweeps for
+    the batch) the chain is still bit-identical and the bins agree to 1e-12."""
+    L = 64
+    reps = [_pair(L, L, L, 1.2 + 0.05 * i, 500 + i)[0] for i in range(4)]  # 4 x 64 tiles
+    for form in (1, 0):
+        big = slsf.Batch(reps, slsf.FP32)
+        one = slsf.Batch([reps[2]], slsf.FP32)
+        for b in (big, one):
+            b.init_angles()
+            b.run(3, 6, 2, hbm_form=form)
+        assert np.array_equal(big.get_theta(2), one.get_theta(0)), form
+        if form == 1:
+            assert np.array_equal(big.all_bins(2), one.all_bins(0))
+        else:
+            assert np.allclose(big.all_bins(2), one.all_bins(0), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["FP64_REPLAY", "FP32"])
+def test_slab_and_per_phase_forms_give_the_same_chains_and_bins(precision):
+    """The two forms of the HBM path (slab sweeps, per-phase sweeps) on a batch
+    of shapes and cluster rounds, also switching form between runs of one
+    batch: angles and counters are bit-identical; the bins, FP64 sums over
+    differently shaped blocks, agree to 1e-12 (relative)."""
+    prec = getattr(slsf, precision)
+    specs = [(48, 48, 48, 1.3, 1, 1), (20, 18, 34, 1.1, 2, 2), (10, 30, 12, 1.6, 3, 0),
+             (64, 64, 64, 1.25, 4, 1)]
+    reps = [_pair(lx, ly, lz, T, seed, ws)[0] for lx, ly, lz, T, seed, ws in specs]
+    out = []
+    for forms in ((1, 1), (2, 2), (1, 2), (2, 1)):
+        b = slsf.Batch(reps, prec)
+        b.init_angles()
+        b.run(3, 6, 2, hbm_form=forms[0])
+        b.run(0, 4, 2, hbm_form=forms[1])
+        out.append((forms, [b.get_theta(i) for i in range(len(reps))],
+                    [b.all_bins(i) for i in range(len(reps))], b.counters()))
+        b.close()
+    ref = out[0]
+    for forms, th, bins, ctr in out[1:]:
+        for i in range(len(reps)):
+            assert np.array_equal(th[i], ref[1][i]), (forms, i)
+            assert np.allclose(bins[i], ref[2][i], rtol=1e-12, atol=1e-12), (forms, i)
+        assert np.array_equal(ctr[0], ref[3][0]) and np.array_equal(ctr[1], ref[3][1])
