@@ -467,45 +467,36 @@ def bench_c4(args, slsf):
     st = b.run(0, sweeps, binsize)
     updates = sum(r.volume for r in reps) * (therm + sweeps)
     value = sum(r.volume for r in reps) * sweeps / st["seconds"]
-    # disorder averages per (L, T): [<m^4>] / [<m^2>]^2 and the helicity moduli,
-    # with jackknife errors over realizations
-    s_m2, s_m4 = slsf.SERIES.index("MagnetizationSquared"), slsf.SERIES.index("MagnetizationFourth")
-    per = {}
+    # disorder averages per (L, T) with jackknife errors over realizations
+    # (slsf_disorder_average: Binder = [<m^4>]/[<m^2>]^2, helicity moduli
+    # averaged over realizations), and the crossings of the Binder ratio and of
+    # L rho_xy between successive sizes (slsf_crossing, jackknifed)
+    means = {L: np.zeros((len(temps), n_real, len(slsf.SERIES))) for L in Ls}
     for i, (L, r, t) in enumerate(keys):
-        bins = b.all_bins(i)
-        ev = slsf.evaluate(bins, float(L ** 3), 1.0 / t)
-        per.setdefault((L, t), []).append((bins[s_m2].mean(), bins[s_m4].mean(),
-                                           ev["RhoXY"][0], ev["RhoZ"][0]))
+        means[L][temps.index(t), r] = b.all_bins(i).mean(axis=1)
     b.close()
     table = {}
-    for (L, t), rows in per.items():
-        a = np.array(rows)
-        m2, m4 = a[:, 0], a[:, 1]
-        g_all = m4.mean() / m2.mean() ** 2
-        n = len(a)
-        loo = [(m4.sum() - m4[k]) / (n - 1) / ((m2.sum() - m2[k]) / (n - 1)) ** 2 for k in range(n)]
-        g, ge = slsf.jackknife(np.array(loo), g_all)
-        rx = a[:, 2]
-        rz = a[:, 3]
-        table.setdefault(str(L), {})[f"{t:.1f}"] = {
-            "binder": g, "binder_err": ge,
-            "rho_xy": rx.mean(), "rho_xy_err": rx.std(ddof=1) / math.sqrt(n),
-            "rho_z": rz.mean(), "rho_z_err": rz.std(ddof=1) / math.sqrt(n)}
+    for L in Ls:
+        for k, t in enumerate(temps):
+            av = slsf.disorder_average(means[L][k], float(L ** 3), 1.0 / t)
+            table.setdefault(str(L), {})[f"{t:.1f}"] = {
+                "binder": av["Binder"][0], "binder_err": av["Binder"][1],
+                "rho_xy": av["RhoXY"][0], "rho_xy_err": av["RhoXY"][1],
+                "rho_z": av["RhoZ"][0], "rho_z_err": av["RhoZ"][1]}
     crossings = {}
     for L1, L2 in zip(Ls[:-1], Ls[1:]):
-        d = [table[str(L1)][f"{t:.1f}"]["binder"] - table[str(L2)][f"{t:.1f}"]["binder"]
-             for t in temps]
-        for k in range(len(temps) - 1):
-            if d[k] == 0 or d[k] * d[k + 1] < 0:
-                tc = temps[k] + (temps[k + 1] - temps[k]) * d[k] / (d[k] - d[k + 1])
-                crossings[f"{L1}/{L2}"] = tc
-                break
+        for name, ev, s1, s2 in (("binder", "Binder", 1.0, 1.0),
+                                 ("L_rho_xy", "RhoXY", float(L1), float(L2))):
+            c = slsf.crossing(temps, means[L1], float(L1 ** 3), means[L2], float(L2 ** 3), ev,
+                              s1, s2)
+            if c is not None:
+                crossings[f"{name} {L1}/{L2}"] = {"T": c[0], "err": c[1]}
     line(value, "site-updates/s", sweeps, therm, 1e3 * st["seconds"] / sweeps,
          {"workload": "uneven batch, autotuned: L in {8,12,16,24,32} x 5 T in [1.1,1.5] x 64 "
                       "realizations (1600 replicas), 1000 + 4000 sweeps, binsize 100",
           "replicas": len(reps)},
          "fp32 (fp64 accumulators)", autotune=tuned, binder_rho=table,
-         binder_crossings=crossings, acceptance=st["acceptance_mean"],
+         crossings=crossings, acceptance=st["acceptance_mean"],
          mean_cluster_size=st["mean_cluster_size"], total_site_updates=updates)
 
 

@@ -335,6 +335,35 @@ int slsf_evaluate(const double* bins, int64_t nbins, double volume, double beta,
 int slsf_jackknife(const double* loo, int64_t nbins, double f_all, double* estimate,
                    double* stderr_out);
 
+/* Disorder averages over realizations (extension; the reference lists every
+ * task and averages nothing across samples).  means[r * SLSF_NUM_SERIES + s] is
+ * realization r's mean of series s (e.g. the mean of its bins).  value[k] is the
+ * mean over realizations of evaluable k of each realization
+ * (src/model.rs:857-888), except the Binder ratio, which is the ratio of
+ * disorder averages [<m^4>] / [<m^2>]^2; error[k] is the jackknife error over
+ * realizations (each realization left out in turn).  Errors: fewer than two
+ * realizations. */
+int slsf_disorder_average(const double* means /* [n_real][SLSF_NUM_SERIES] */,
+                          int32_t n_real, double volume, double beta,
+                          double* value /* [SLSF_NUM_EVALS] */,
+                          double* error /* [SLSF_NUM_EVALS] */);
+
+/* Crossing temperature of the disorder average of evaluable `eval`, scaled by
+ * scale_a and scale_b (1 for the Binder ratio, L for the helicity moduli, whose
+ * L rho curves cross at a 3D XY transition), between two lattice sizes a and b
+ * (extension): means_a[(t * n_real + r) * SLSF_NUM_SERIES + s] as in
+ * slsf_disorder_average for temperature temps[t] (increasing), and the same for
+ * b.  The first t with d = scale_a f_a - scale_b f_b changing sign (or zero)
+ * between temps[t] and temps[t + 1] gives the crossing by linear interpolation;
+ * *found = 0 when the curves do not cross.  The error is the jackknife over
+ * realizations: realization r is left out of both sizes at every temperature
+ * and the crossing found again (NaN when some leave-one-out curves do not
+ * cross).  Errors: fewer than two temperatures or realizations. */
+int slsf_crossing(const double* temps, int32_t n_temps, int32_t n_real, int32_t eval,
+                  const double* means_a, double volume_a, double scale_a,
+                  const double* means_b, double volume_b, double scale_b, int32_t* found,
+                  double* t_cross, double* t_error);
+
 /* ======================================================================== */
 /* Serving: continuous batching of independent jobs                          */
 /* ======================================================================== */

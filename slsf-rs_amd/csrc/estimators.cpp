@@ -49,6 +49,63 @@ double eval_one(int k, const double* m, double volume, double beta) {
   }
 }
 
+// Disorder-averaged evaluable k over the realizations r with keep[r] (all
+// when keep is null): the mean of the per-realization evaluables, except the
+// Binder ratio, the ratio of the disorder averages of <m^4> and <m^2>^2.
+double disorder_eval(int k, const double* means, int n_real, const std::vector<char>* keep,
+                     double volume, double beta) {
+  double sum = 0.0, m2 = 0.0, m4 = 0.0;
+  int n = 0;
+  for (int r = 0; r < n_real; ++r) {
+    if (keep && !(*keep)[r]) continue;
+    const double* m = means + (size_t)r * SLSF_NUM_SERIES;
+    if (k == SLSF_EVAL_BINDER) {
+      m2 += m[SLSF_SERIES_M2];
+      m4 += m[SLSF_SERIES_M4];
+    } else {
+      sum += eval_one(k, m, volume, beta);
+    }
+    ++n;
+  }
+  if (k == SLSF_EVAL_BINDER) {
+    m2 /= (double)n;
+    m4 /= (double)n;
+    return m4 / (m2 * m2);
+  }
+  return sum / (double)n;
+}
+
+// jackknife standard error of the leave-one-out values loo[0..n)
+double jackknife_error(const std::vector<double>& loo) {
+  const int64_t n = (int64_t)loo.size();
+  const double m = mean(loo.data(), n);
+  double v = 0.0;
+  for (double x : loo) v += (x - m) * (x - m);
+  return std::sqrt((double)(n - 1) / (double)n * v);
+}
+
+// first sign change of a - b over the temperature grid, linear interpolation
+bool first_crossing(const double* t, const std::vector<double>& a, const std::vector<double>& b,
+                    double* tc) {
+  const size_t n = a.size();
+  for (size_t k = 0; k + 1 < n; ++k) {
+    const double d0 = a[k] - b[k], d1 = a[k + 1] - b[k + 1];
+    if (d0 == 0.0) {
+      *tc = t[k];
+      return true;
+    }
+    if ((d0 < 0.0) != (d1 < 0.0) && d1 != d0) {
+      *tc = t[k] + (t[k + 1] - t[k]) * d0 / (d0 - d1);
+      return true;
+    }
+  }
+  if (n > 0 && a[n - 1] == b[n - 1]) {
+    *tc = t[n - 1];
+    return true;
+  }
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -132,6 +189,63 @@ int slsf_evaluate(const double* bins, int64_t nbins, double volume, double beta,
     for (int64_t i = 0; i < nbins; ++i) v += (l[i] - lm) * (l[i] - lm);
     error[k] = std::sqrt((double)(nbins - 1) / (double)nbins * v);
   }
+  return SLSF_OK;
+}
+
+int slsf_disorder_average(const double* means, int32_t n_real, double volume, double beta,
+                          double* value, double* error) {
+  clear_error();
+  if (!means || !value) return fail(SLSF_E_INVALID, "null argument");
+  if (n_real < 2) return fail(SLSF_E_INVALID, "a disorder average needs at least two realizations");
+  std::vector<char> keep((size_t)n_real, 1);
+  std::vector<double> loo((size_t)n_real);
+  for (int k = 0; k < SLSF_NUM_EVALS; ++k) {
+    value[k] = disorder_eval(k, means, n_real, nullptr, volume, beta);
+    if (!error) continue;
+    for (int r = 0; r < n_real; ++r) {
+      keep[r] = 0;
+      loo[r] = disorder_eval(k, means, n_real, &keep, volume, beta);
+      keep[r] = 1;
+    }
+    error[k] = jackknife_error(loo);
+  }
+  return SLSF_OK;
+}
+
+int slsf_crossing(const double* temps, int32_t n_temps, int32_t n_real, int32_t eval,
+                  const double* means_a, double volume_a, double scale_a,
+                  const double* means_b, double volume_b, double scale_b, int32_t* found,
+                  double* t_cross, double* t_error) {
+  clear_error();
+  if (!temps || !means_a || !means_b || !found || !t_cross)
+    return fail(SLSF_E_INVALID, "null argument");
+  if (n_temps < 2) return fail(SLSF_E_INVALID, "a crossing needs at least two temperatures");
+  if (n_real < 2) return fail(SLSF_E_INVALID, "a crossing needs at least two realizations");
+  if (eval < 0 || eval >= SLSF_NUM_EVALS) return fail(SLSF_E_INVALID, "unknown evaluable");
+  const size_t per_t = (size_t)n_real * SLSF_NUM_SERIES;
+  std::vector<double> a((size_t)n_temps), b((size_t)n_temps);
+  auto curves = [&](const std::vector<char>* keep) {
+    for (int t = 0; t < n_temps; ++t) {
+      const double beta = 1.0 / temps[t];
+      a[t] = scale_a * disorder_eval(eval, means_a + t * per_t, n_real, keep, volume_a, beta);
+      b[t] = scale_b * disorder_eval(eval, means_b + t * per_t, n_real, keep, volume_b, beta);
+    }
+  };
+  curves(nullptr);
+  double tc = 0.0;
+  *found = first_crossing(temps, a, b, &tc) ? 1 : 0;
+  *t_cross = *found ? tc : NAN;
+  if (t_error) *t_error = NAN;
+  if (!*found || !t_error) return SLSF_OK;
+  std::vector<char> keep((size_t)n_real, 1);
+  std::vector<double> loo((size_t)n_real);
+  for (int r = 0; r < n_real; ++r) {
+    keep[r] = 0;
+    curves(&keep);
+    keep[r] = 1;
+    if (!first_crossing(temps, a, b, &loo[r])) return SLSF_OK;  // error stays NaN
+  }
+  *t_error = jackknife_error(loo);
   return SLSF_OK;
 }
 

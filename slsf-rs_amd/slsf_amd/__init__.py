@@ -187,6 +187,11 @@ _SIGS = {
     "slsf_evaluate": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                      _dp, _dp]),
     "slsf_jackknife": (ctypes.c_int, [_dp, ctypes.c_int64, ctypes.c_double, _dp, _dp]),
+    "slsf_disorder_average": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_double,
+                                             ctypes.c_double, _dp, _dp]),
+    "slsf_crossing": (ctypes.c_int, [_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp,
+                                     ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double,
+                                     ctypes.c_double, _i32p, _dp, _dp]),
     "slsf_job_from_toml": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_void_p)]),
     "slsf_job_destroy": (None, [ctypes.c_void_p]),
     "slsf_job_name": (ctypes.c_char_p, [ctypes.c_void_p]),
@@ -500,6 +505,38 @@ def jackknife(loo, f_all: float):
     _check(_lib.slsf_jackknife(_ptr(a, ctypes.c_double), len(a), f_all, ctypes.byref(est),
                                ctypes.byref(err)))
     return est.value, err.value
+
+
+def disorder_average(means, volume: float, beta: float) -> dict:
+    """Disorder averages of the evaluables over realizations with jackknife
+    errors; means[r][s] = realization r's mean of series s (slsf_disorder_average)."""
+    m = np.ascontiguousarray(means, dtype=np.float64)
+    assert m.ndim == 2 and m.shape[1] == len(SERIES)
+    v = np.zeros(len(EVALS))
+    e = np.zeros(len(EVALS))
+    _check(_lib.slsf_disorder_average(_ptr(m, ctypes.c_double), m.shape[0], volume, beta,
+                                      _ptr(v, ctypes.c_double), _ptr(e, ctypes.c_double)))
+    return {n: (v[i], e[i]) for i, n in enumerate(EVALS)}
+
+
+def crossing(temps, means_a, volume_a: float, means_b, volume_b: float,
+             evaluable: str = "Binder", scale_a: float = 1.0, scale_b: float = 1.0):
+    """Crossing temperature of scale * (a disorder-averaged evaluable) between
+    two sizes with its jackknife error over realizations (slsf_crossing);
+    means_x[t][r][s].  Returns (temperature, error) or None when the curves do
+    not cross."""
+    t = np.ascontiguousarray(temps, dtype=np.float64)
+    a = np.ascontiguousarray(means_a, dtype=np.float64)
+    b = np.ascontiguousarray(means_b, dtype=np.float64)
+    assert a.shape == b.shape and a.shape[0] == len(t) and a.shape[2] == len(SERIES)
+    found = ctypes.c_int32()
+    tc, te = ctypes.c_double(), ctypes.c_double()
+    _check(_lib.slsf_crossing(_ptr(t, ctypes.c_double), len(t), a.shape[1],
+                              EVALS.index(evaluable), _ptr(a, ctypes.c_double), volume_a,
+                              scale_a, _ptr(b, ctypes.c_double), volume_b, scale_b,
+                              ctypes.byref(found),
+                              ctypes.byref(tc), ctypes.byref(te)))
+    return (tc.value, te.value) if found.value else None
 
 
 # ---- serving -----------------------------------------------------------------------

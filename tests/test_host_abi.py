@@ -410,3 +410,78 @@ def test_jackknife_error_matches_the_spread_over_independent_datasets():
     for k in vals:
         spread = np.std(vals[k], ddof=1)
         assert abs(np.mean(errs[k]) / spread - 1) < 0.1, (k, np.mean(errs[k]), spread)
+
+
+def test_disorder_average_against_numpy():
+    """Disorder averages over realizations: the mean of per-realization
+    evaluables with the standard error of the mean (a linear average's
+    jackknife error), and the Binder ratio as the ratio of disorder averages
+    [<m^4>]/[<m^2>]^2 with a leave-one-realization-out jackknife."""
+    rng = np.random.default_rng(21)
+    V, beta, n = 4096.0, 1 / 1.2, 24
+    means = _synthetic_bins(rng, n).T  # [n_real][13]
+    out = slsf.disorder_average(means, V, beta)
+    per = [_evaluables_numpy(m, V, beta) for m in means]
+    for name in ("Energy", "Chi", "SpecificHeat", "RhoXY", "RhoZ", "Magnetization"):
+        x = np.array([p[name] for p in per])
+        assert abs(out[name][0] - x.mean()) <= 1e-12 * max(1.0, abs(x.mean())), name
+        assert abs(out[name][1] - x.std(ddof=1) / math.sqrt(n)) <= 1e-10 * max(1.0, x.std()), name
+    m2, m4 = means[:, 2], means[:, 3]
+    g = m4.mean() / m2.mean() ** 2
+    loo = np.array([np.delete(m4, r).mean() / np.delete(m2, r).mean() ** 2 for r in range(n)])
+    err = math.sqrt((n - 1) / n * np.sum((loo - loo.mean()) ** 2))
+    assert abs(out["Binder"][0] - g) <= 1e-14 and abs(out["Binder"][1] - err) <= 1e-12
+    with pytest.raises(slsf.SlsfError, match="at least two realizations"):
+        slsf.disorder_average(means[:1], V, beta)
+
+
+def _binder_means(temps, n_real, slope, noise, rng, tc=1.3, b0=1.2):
+    """Per-realization series means whose disorder-averaged Binder ratio is
+    b0 - slope (T - tc) plus realization noise: m^2 = 1, m^4 = that ratio."""
+    out = np.zeros((len(temps), n_real, len(slsf.SERIES)))
+    out[:, :, 2] = 1.0
+    for t, T in enumerate(temps):
+        out[t, :, 3] = b0 - slope * (T - tc) + noise * rng.standard_normal(n_real)
+    return out
+
+
+def test_crossing_of_lines_with_a_known_intersection():
+    temps = np.array([1.1, 1.2, 1.25, 1.35, 1.5])
+    rng = np.random.default_rng(4)
+    a = _binder_means(temps, 16, 0.5, 0.0, rng)
+    b = _binder_means(temps, 16, 0.8, 0.0, rng)
+    t, e = slsf.crossing(temps, a, 512.0, b, 4096.0)
+    assert abs(t - 1.3) <= 1e-12 and e <= 1e-12
+    # curves that do not cross
+    c = _binder_means(temps, 16, 0.5, 0.0, rng, b0=1.5)
+    assert slsf.crossing(temps, a, 512.0, c, 4096.0) is None
+    # a crossing exactly on a grid temperature
+    grid = np.array([1.2, 1.3, 1.4])
+    t2, _ = slsf.crossing(grid, _binder_means(grid, 16, 0.5, 0.0, rng), 512.0,
+                          _binder_means(grid, 16, 0.8, 0.0, rng), 4096.0)
+    assert abs(t2 - 1.3) <= 1e-12
+    with pytest.raises(slsf.SlsfError, match="at least two temperatures"):
+        slsf.crossing(temps[:1], a[:1], 512.0, b[:1], 4096.0)
+    # scaled curves (the L rho crossings): 1.02 (1.2 - 0.5 d) = 1.2 - 0.8 d at
+    # d = T - 1.3 = -0.024 / 0.29
+    t3, _ = slsf.crossing(temps, a, 512.0, b, 4096.0, scale_a=1.02, scale_b=1.0)
+    assert abs(t3 - (1.3 - 0.024 / 0.29)) <= 1e-12
+
+
+def test_crossing_jackknife_error_matches_the_spread_over_datasets():
+    """With realization noise the crossing scatters around 1.3; its jackknife
+    error over realizations matches the standard deviation of the crossing over
+    300 independent synthetic datasets (within 15%)."""
+    temps = np.array([1.1, 1.2, 1.3, 1.4, 1.5])
+    rng = np.random.default_rng(9)
+    ts, es = [], []
+    for _ in range(300):
+        a = _binder_means(temps, 32, 0.5, 0.02, rng)
+        b = _binder_means(temps, 32, 0.9, 0.02, rng)
+        res = slsf.crossing(temps, a, 512.0, b, 4096.0)
+        assert res is not None
+        ts.append(res[0])
+        es.append(res[1])
+    spread = np.std(ts, ddof=1)
+    assert abs(np.mean(ts) - 1.3) < 4 * spread / math.sqrt(300) + 1e-3
+    assert abs(np.mean(es) / spread - 1) < 0.15, (np.mean(es), spread)
