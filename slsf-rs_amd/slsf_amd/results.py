@@ -32,7 +32,13 @@ enabled also get the reference's ``CorrX_r*``, ``CorrY_r*``, ``CorrXY_r*``,
 ``CorrXY_z{z}_r*`` and ``CorrZ_r*`` series (src/model.rs:410-435): measured on
 the device on every ``correlation_interval``-th measurement (the first
 included), the batch stepping to each of those sweeps, and binned like every
-series.  Only the ``_ll_*`` timers are absent.
+series.  The reference's timers ``_ll_sweep_time`` and ``_ll_measure_time``
+(src/model.rs:380,398,403,438-439) are binned like the series too: a batch runs
+all its tasks' sweeps in the same launches, so a task's sweep time is its share
+of the site updates of the run's wall time per sweep (the measurement on the
+device is fused into the sweep kernels and counted there), and its measure
+time is the host time spent finishing its measurements (reading its bins back,
+taking its correlation samples) per measurement.
 """
 from __future__ import annotations
 
@@ -107,6 +113,43 @@ def _corr_names(rxy: int, rz: int, lz: int) -> list:
             [f"CorrZ_r{k}" for k in range(1, rz + 1)])
 
 
+class _Timers:
+    """Wall time of a batch's measured runs, per measured sweep, and host time
+    spent per task on finishing its measurements."""
+
+    def __init__(self, tasks):
+        vol = np.array([float(t["l_x"] * t["l_y"] * t["l_z"]) for t in tasks])
+        self.share = vol / vol.sum() if len(vol) else vol
+        self.segments = []  # (measured sweeps, seconds) per measured run
+        self.measure_seconds = np.zeros(len(tasks))
+
+    def run(self, b: Batch, n: int, binsize: int, **kw):
+        t0 = time.perf_counter()
+        b.run(0, n, binsize, **kw)
+        self.segments.append((n, time.perf_counter() - t0))
+
+    def sweep_times(self, k: int, measured: int) -> np.ndarray:
+        """Task k's time per measured sweep, in measurement order."""
+        per = [np.full(n, sec / n * self.share[k]) for n, sec in self.segments if n > 0]
+        v = np.concatenate(per) if per else np.zeros(0)
+        return v[:measured] if len(v) >= measured else np.concatenate(
+            [np.full(measured - len(v), np.nan), v])
+
+    def state(self) -> dict:
+        return {"timer_segments": np.array(self.segments, dtype=np.float64).reshape(-1, 2),
+                "timer_measure": self.measure_seconds}
+
+    def restore(self, arrays) -> None:
+        if arrays is not None and "timer_segments" in arrays:
+            self.segments = [(int(n), float(t)) for n, t in arrays["timer_segments"]]
+            self.measure_seconds = np.array(arrays["timer_measure"], dtype=np.float64)
+
+
+def _bin_means(values: np.ndarray, binsize: int) -> np.ndarray:
+    nb = len(values) // binsize
+    return values[:nb * binsize].reshape(nb, binsize).mean(axis=1) if nb else np.zeros(0)
+
+
 class _Correlations:
     """Correlation samples of the tasks that ask for them, taken on the
     measured sweeps with index m % interval == 0 (m counts from 0)."""
@@ -120,28 +163,33 @@ class _Correlations:
         self.interval = ivals.pop() if ivals else 0
         self.samples = [[] for _ in tasks]
 
-    def sample(self, b: Batch):
+    def sample(self, b: Batch, timers: "_Timers | None" = None):
         for k, ((rxy, rz, _), on) in enumerate(zip(self.spec, self.on)):
             if not on:
                 continue
+            t0 = time.perf_counter()
             c = b.correlations(k, -1, rxy, rz)
             self.samples[k].append(np.concatenate(
                 [c["corr_x"], c["corr_y"], c["corr_xy"], c["corr_xy_by_z"].ravel(), c["corr_z"]]))
+            if timers is not None:
+                timers.measure_seconds[k] += time.perf_counter() - t0
 
-    def run(self, b: Batch, start: int, n: int, binsize: int):
+    def run(self, b: Batch, start: int, n: int, binsize: int, timers: "_Timers | None" = None,
+            **kw):
         """Measured sweeps start .. start+n-1 (indices of measurements)."""
+        timers = timers if timers is not None else _Timers([])
         if not self.interval:
-            b.run(0, n, binsize)
+            timers.run(b, n, binsize, **kw)
             return
         done = 0
         while done < n:
             m = start + done
             nxt = m + (-m) % self.interval  # next measurement taking correlations
             if nxt >= start + n:
-                b.run(0, start + n - m, binsize)
+                timers.run(b, start + n - m, binsize, **kw)
                 break
-            b.run(0, nxt - m + 1, binsize)
-            self.sample(b)
+            timers.run(b, nxt - m + 1, binsize, **kw)
+            self.sample(b, timers)
             done = nxt - start + 1
 
     def state(self) -> dict:
@@ -175,12 +223,15 @@ class _Correlations:
 
 
 def _task_results(b: Batch, idx, tasks, measured: int, binsize: int,
-                  corr: "_Correlations | None" = None) -> list:
+                  corr: "_Correlations | None" = None, timers: "_Timers | None" = None) -> list:
     """ThetaTaskResult entries of a batch that has measured `measured` sweeps."""
     out = []
     acceptance = b.acceptance()
     for k, (i, t) in enumerate(zip(idx, tasks)):
+        t0 = time.perf_counter()
         bins = b.all_bins(k) if measured else np.zeros((len(SERIES), 0))
+        if timers is not None:
+            timers.measure_seconds[k] += time.perf_counter() - t0
         obs = {name: _binned(bins[s], binsize) for s, name in enumerate(SERIES)}
         if bins.shape[1] > 0:
             ev = evaluate(bins, float(t["l_x"] * t["l_y"] * t["l_z"]), 1.0 / t["temperature"])
@@ -197,6 +248,14 @@ def _task_results(b: Batch, idx, tasks, measured: int, binsize: int,
                 obs[name] = _binned(v, binsize)
                 if len(v):
                     mbins[name] = v
+        if timers is not None and measured:
+            for name, v in (("_ll_sweep_time", timers.sweep_times(k, measured)),
+                            ("_ll_measure_time",
+                             np.full(measured, timers.measure_seconds[k] / measured))):
+                tb = _bin_means(v, binsize)
+                obs[name] = _binned(tb, binsize)
+                if len(tb):
+                    mbins[name] = tb
         entry = {"task": _task_json(t)}
         if i != 0:
             entry["task_index"] = int(i)  # serialised only when nonzero
@@ -216,15 +275,17 @@ def run_job(job: Job, rank: int = 0, world_size: int = 1, precision: int = FP32,
         return result
     therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
     corr = _Correlations(tasks)
+    timers = _Timers(tasks)
     try:
         b.init_angles()
         if therm:
             b.run(therm, 0, 1, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
         if sweeps and corr.interval:
-            corr.run(b, 0, sweeps, binsize)
+            corr.run(b, 0, sweeps, binsize, timers)
         elif sweeps:
-            b.run(0, sweeps, binsize, sweeps_per_launch=sweeps_per_launch, use_graph=use_graph)
-        result["tasks"] = _task_results(b, idx, tasks, sweeps, binsize, corr)
+            timers.run(b, sweeps, binsize, sweeps_per_launch=sweeps_per_launch,
+                       use_graph=use_graph)
+        result["tasks"] = _task_results(b, idx, tasks, sweeps, binsize, corr, timers)
     finally:
         b.close()
     return result
@@ -235,13 +296,19 @@ def checkpoint_path(job_name: str, rank: int, output_dir: str | None = None) -> 
     return os.path.join(measurement_dir(job_name, output_dir), f"rank{rank}.checkpoint")
 
 
-def _save_image(b: Batch, path: str, corr: "_Correlations | None" = None) -> None:
+def _save_image(b: Batch, path: str, corr: "_Correlations | None" = None,
+                timers: "_Timers | None" = None) -> None:
     parent = os.path.dirname(path)
     if parent:
         os.makedirs(parent, exist_ok=True)
-    if corr is not None and corr.interval:  # host-side correlation samples beside the image
+    side = {}  # host-side correlation samples and timers beside the image
+    if corr is not None and corr.interval:
+        side.update(corr.state())
+    if timers is not None:
+        side.update(timers.state())
+    if side:
         tmp = path + ".corr.tmp.npz"
-        np.savez(tmp, **corr.state())
+        np.savez(tmp, **side)
         os.replace(tmp, path + ".corr.npz")
     tmp = path + ".tmp"
     b.checkpoint().tofile(tmp)
@@ -279,19 +346,18 @@ def run_job_resumable(job: Job, path: str, rank: int = 0, world_size: int = 1,
         return result, summary
     therm, sweeps, binsize = (int(tasks[0][k]) for k in ("thermalization", "sweeps", "binsize"))
     corr = _Correlations(tasks)
+    timers = _Timers(tasks)
     try:
         if os.path.exists(path):
             b.restore(np.fromfile(path, dtype=np.uint8))
         else:
             b.init_angles()
         done = int(b.counters()[0][0])
+        side = path + ".corr.npz"
+        arrays = dict(np.load(side)) if os.path.exists(side) else None
         if corr.interval:
-            side = path + ".corr.npz"
-            if os.path.exists(side):
-                with np.load(side) as f:
-                    corr.restore(f, max(0, done - therm))
-            else:
-                corr.restore(None, max(0, done - therm))
+            corr.restore(arrays, max(0, done - therm))
+        timers.restore(arrays)
         last = time.monotonic()
         while done < therm + sweeps:
             if time.monotonic() - started >= run_time:
@@ -302,14 +368,15 @@ def run_job_resumable(job: Job, path: str, rank: int = 0, world_size: int = 1,
                 b.run(n, 0, 1)
             else:
                 n = min(chunk_sweeps, therm + sweeps - done)
-                corr.run(b, done - therm, n, binsize)
+                corr.run(b, done - therm, n, binsize, timers)
             done += n
             if time.monotonic() - last >= checkpoint_time:
-                _save_image(b, path, corr)
+                _save_image(b, path, corr, timers)
                 last = time.monotonic()
-        _save_image(b, path, corr)
+        _save_image(b, path, corr, timers)
         summary["checkpoint_paths"] = [path]
-        result["tasks"] = _task_results(b, idx, tasks, max(0, done - therm), binsize, corr)
+        result["tasks"] = _task_results(b, idx, tasks, max(0, done - therm), binsize, corr,
+                                        timers)
     finally:
         b.close()
     summary["elapsed_seconds"] = time.monotonic() - started

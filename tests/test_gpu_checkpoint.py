@@ -111,8 +111,15 @@ def test_resumable_run_stops_and_resumes_to_the_same_result(tmp_path, monkeypatc
     for tf, td in zip(full["tasks"], done["tasks"]):
         assert td["measurements"] == 40
         for name, est in tf["observables"].items():
+            if name.startswith("_ll_"):
+                # wall-clock series: same bins, values from this process's clock
+                continue
             assert td["observables"][name]["mean"] == est["mean"], name
         for name, v in tf["measurement_bins"].items():
+            if name.startswith("_ll_"):
+                w = td["measurement_bins"][name]
+                assert w.shape == v.shape and np.all(np.isfinite(w)) and np.all(w >= 0), name
+                continue
             assert np.array_equal(td["measurement_bins"][name], v), name
         assert math.isclose(td["acceptance"], tf["acceptance"], rel_tol=1e-12)
     # a finished checkpoint resumes to the same result without running again

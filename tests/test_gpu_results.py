@@ -72,7 +72,8 @@ def test_run_job_measurement_files(tmp_path):
         # the 13 series and, correlations being on by default (interval 1), the Corr series
         t = tr["task"]
         corr = R._corr_names(t["l_x"] // 2, t["l_z"] // 2, t["l_z"])
-        assert sorted(tr["measurement_bins"]) == sorted(list(slsf.SERIES) + corr)
+        assert sorted(tr["measurement_bins"]) == sorted(
+            list(slsf.SERIES) + corr + ["_ll_sweep_time", "_ll_measure_time"])
         assert all(len(v) == 6 for v in tr["measurement_bins"].values())
     h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
     if os.path.exists(h5dump):
@@ -116,3 +117,66 @@ def test_correlation_series_follow_the_reference_cadence():
         assert tr["observables"][name]["mean"] == float(np.mean(s[:, j])), name
         assert tr["observables"][name]["internal_bin_len"] == 2
     assert np.array_equal(b.all_bins(0)[0], tr["measurement_bins"]["Energy"])
+
+
+JOB_ORACLE = ('name = "oracle_unit"\n[model]\nL=[6,8]\nT=[1.2,1.7]\nsamples=2\nbase_seed=5\n'
+              'disorder="uniform"\nj_xy_min=0.6\nj_xy_max=1.8\nj_z=1.0\n'
+              '[run]\nsweeps=24\nthermalization=6\nbinsize=4\n'
+              '[measure]\ncorr_rmax_xy=[2,3]\ncorr_rmax_z=[2,2]\ncorrelation_interval=2\n')
+
+
+def test_run_job_fp64_replay_equals_the_oracle_series():
+    """A TOML job through run_job in FP64 replay: every task's binned series,
+    its correlation series (on measurements 0, 2, 4, ...) and its estimates
+    equal the oracle's replay chain of the same task binned alike (1e-11); the
+    timer series (_ll_sweep_time, _ll_measure_time) are positive and binned
+    alike."""
+    import oracle_lib as orc
+    job = slsf.Job(JOB_ORACLE)
+    res = R.run_job(job, precision=slsf.FP64_REPLAY)
+    assert len(res["tasks"]) == 8
+    for k, tr in enumerate(res["tasks"]):
+        t = tr["task"]
+        rep = job.replica(k)
+        therm, sweeps, bs = 6, 24, 4
+        lat = orc.Lattice(rep.l_x, rep.l_y, rep.l_z)
+        lat.set_couplings(rep.j_xy, rep.j_z)
+        lat.init_keyed(rep.seed)
+        rxy, rz, interval = R._corr_spec(t)
+        series, corr = [], []
+        kw = 0
+        for sw in range(therm + sweeps):
+            lat.metro_replay(rep.temperature, rep.proposal_width, rep.seed, sw)
+            for _ in range(rep.wolff_steps):
+                lat.wolff_replay(rep.temperature, rep.seed, kw)
+                kw += 1
+            if sw < therm:
+                continue
+            o = lat.observables()
+            e, m2 = o["energy"], o["magnetization_squared"]
+            series.append([e, e * e, m2, m2 * m2, o["cos_x"], o["cos_y"], o["cos_z"],
+                           o["sin_x"], o["sin_y"], o["sin_z"], o["sin_x"] ** 2,
+                           o["sin_y"] ** 2, o["sin_z"] ** 2])
+            if (sw - therm) % interval == 0:
+                c = lat.correlations(-1, rxy, rz)
+                corr.append(np.concatenate([c["corr_x"], c["corr_y"], c["corr_xy"],
+                                            c["corr_xy_by_z"].ravel(), c["corr_z"]]))
+        expect = np.array(series).reshape(sweeps // bs, bs, 13).mean(axis=1).T
+        for sidx, name in enumerate(slsf.SERIES):
+            got = tr["measurement_bins"][name]
+            assert np.allclose(got, expect[sidx], rtol=1e-11, atol=1e-12), (k, name)
+            assert abs(tr["observables"][name]["mean"] - expect[sidx].mean()) <= \
+                1e-11 * max(1.0, abs(expect[sidx].mean())), (k, name)
+        corr = np.array(corr)
+        nb = len(corr) // bs
+        cexp = corr[:nb * bs].reshape(nb, bs, -1).mean(axis=1)
+        for j, name in enumerate(R._corr_names(rxy, rz, rep.l_z)):
+            assert np.allclose(tr["measurement_bins"][name], cexp[:, j], rtol=1e-11,
+                               atol=1e-12), (k, name)
+        ev = slsf.evaluate(expect, float(rep.volume), 1.0 / rep.temperature)
+        for name in ("RhoXY", "RhoZ", "Chi", "SpecificHeat", "Binder"):
+            assert abs(tr["observables"][name]["mean"] - ev[name][0]) <= \
+                1e-10 * max(1.0, abs(ev[name][0])), (k, name)
+        for name in ("_ll_sweep_time", "_ll_measure_time"):
+            v = tr["measurement_bins"][name]
+            assert len(v) == sweeps // bs and np.all(v > 0) and np.all(np.isfinite(v)), name
