@@ -387,7 +387,12 @@ typedef struct {
   uint64_t seed;             /* key of the job's uniforms */
   int64_t thermalization, sweeps;
   int32_t binsize;           /* raised if sweeps/binsize exceeds the pool's bin capacity */
-  int32_t reserved;
+  /* Correlation series (src/model.rs:360-365,410-435): with a radius above 0
+   * the job measures C(r) on the measurements m with m % interval == 0
+   * (interval below 1 counts as 1) for r <= min(rmax_xy, l/2) in plane and
+   * r <= min(rmax_z, l/2) along z, and bins the samples binsize to a bin. */
+  int32_t correlation_interval;
+  int32_t correlation_rmax_xy, correlation_rmax_z;
 } slsf_job_request;
 
 typedef struct {
@@ -398,6 +403,9 @@ typedef struct {
   double value[SLSF_NUM_EVALS], error[SLSF_NUM_EVALS]; /* slsf_evaluate of the bins */
   double series_mean[SLSF_NUM_SERIES];
   int64_t site_updates;
+  /* correlation series: radii, complete bins and series count
+   * (3 rxy + l rxy + rz); the bins are taken with slsf_server_correlations */
+  int32_t corr_rxy, corr_rz, corr_nbins, corr_nseries;
 } slsf_job_result;
 
 typedef struct {
@@ -420,6 +428,14 @@ int slsf_server_step(slsf_server* server, int32_t* finished);
 /* Drains finished results (oldest first). */
 int slsf_server_poll(slsf_server* server, slsf_job_result* out, int32_t cap, int32_t* n);
 int slsf_server_get_stats(const slsf_server* server, slsf_server_stats* out);
+/* Bin means of a finished job's correlation series, [corr_nseries][corr_nbins],
+ * series in the order CorrX_r1.., CorrY_r1.., CorrXY_r1.., CorrXY_z0_r1..
+ * (layer-major), CorrZ_r1.. (src/model.rs:413-435).  out == NULL: *count
+ * receives the number of doubles and the series stay; otherwise they are
+ * copied (cap >= that number) and released.  The server holds a finished
+ * job's series until this call takes them. */
+int slsf_server_correlations(slsf_server* server, int64_t job_id, double* out, int64_t cap,
+                             int64_t* count);
 /* Bytes one slot of size l occupies (the admission footprint). */
 int64_t slsf_server_slot_bytes(int32_t l);
 /* Free and total memory of the current device (hipMemGetInfo), for choosing a

@@ -60,6 +60,12 @@ template <typename R> void launch_observables(const GlobalState& g, hipStream_t 
 // rows = ly * lz of that replica (sizes the grid)
 template <typename R> void launch_corr(const GlobalState& g, int replica, int64_t rows, int rxy, int rz,
                                        double* out_xy, double* out_z, hipStream_t s);
+// k_corr_planes: out[b] (b < rxy lz, shift b / lz + 1, layer b % lz) = that
+// layer's x sum, out[rxy lz + b] its y sum, out[2 rxy lz + k] the z sum of
+// shift k + 1; the sums of launch_corr's rows in the host fold's order
+// (2 rxy lz + rz doubles; ly, lz of that replica)
+template <typename R> void launch_corr_planes(const GlobalState& g, int replica, int ly, int lz,
+                                              int rxy, int rz, double* out, hipStream_t s);
 // zeroes the accepted-move and cluster-site tallies
 void launch_clear_counters(const GlobalState& g, hipStream_t s);
 // opens (open = 1) or closes the run window of every replica (RepDev

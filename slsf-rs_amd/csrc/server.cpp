@@ -9,6 +9,12 @@
 // window end.  Pools are driven on separate streams, so jobs of different sizes
 // overlap on the device.  Per-replica keyed uniforms and per-replica reductions
 // make a job's bins independent of the mix it ran in.
+//
+// A job that asks for correlation series is sampled on the reference's
+// cadence (after the sweeps of the measurements m with m % interval == 0):
+// a pool's chunk ends at the earliest such sweep of its active slots, and
+// after it k_corr_planes forms the sampled slots' plane sums, which the host
+// folds and bins per job.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,13 +74,33 @@ struct SplitMix64 {
   double f64() { return (double)(next() >> 11) * 0x1.0p-53; }
 };
 
+// plane sums of one correlation sample: 2 rxy l + rz doubles with rxy, rz <= l/2
+int64_t corr_plane_doubles(int L) { return (int64_t)L * L + L; }
+
 struct Job {
   int64_t id;
   slsf_job_request req;
   int32_t binsize;
   double submit_s, start_s = -1.0;
   int slot = -1;
+  // correlation series: radii, cadence, the sweep count of the next sample
+  // (0: none left), the open bin's sums and the complete bins' means
+  int32_t rxy = 0, rz = 0, interval = 1;
+  int64_t next_sample = 0;
+  int32_t in_bin = 0;
+  std::vector<double> bin_sum, bin_means;
+  bool corr() const { return rxy > 0 || rz > 0; }
+  int nseries() const { return 3 * rxy + req.l * rxy + rz; }
 };
+
+// Sweep count after which measurement m = k * interval is taken (the first
+// such count above `done`), or 0 when the window has no further sample.
+int64_t next_sample_after(const Job& j, int64_t done) {
+  const int64_t therm = j.req.thermalization;
+  const int64_t m = std::max<int64_t>(0, done - therm);  // measurements taken
+  const int64_t k = (m + j.interval - 1) / j.interval * j.interval;
+  return k < j.req.sweeps ? therm + k + 1 : 0;
+}
 
 struct Pool {
   int L = 0;
@@ -94,6 +120,9 @@ struct Pool {
   ResidentGroup group;            // active slots of a resident pool
   bool running = false;
   bool cs_stale = false;          // slab chunks left the trig cache behind the angles
+  std::vector<int64_t> done;      // each slot's sweep count after the last chunk
+  double* corr_planes = nullptr;  // one correlation sample's plane sums
+  std::vector<double> corr_host;
 
   ~Pool() {
     for (void* p : allocs) (void)hipFree(p);
@@ -126,6 +155,7 @@ struct slsf_server {
   std::deque<Job> queue;
   std::map<int64_t, Job> running;
   std::deque<slsf_job_result> done;
+  std::map<int64_t, std::vector<double>> corr_done;  // finished jobs' series, [series][bin]
   int64_t next_id = 1, finished = 0, site_updates = 0;
 };
 
@@ -133,8 +163,9 @@ namespace {
 
 int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   const int64_t per = slot_bytes(L);
-  const int64_t room = s->budget - s->used;
-  const int cap = (int)std::min<int64_t>(s->max_slots, room / per);
+  const int64_t planes = corr_plane_doubles(L) * (int64_t)sizeof(double);
+  const int64_t room = s->budget - s->used - std::max<int64_t>(planes, 256);
+  const int cap = (int)std::min<int64_t>(s->max_slots, std::max<int64_t>(room, 0) / per);
   if (cap <= 0) return SLSF_OK;  // not now: the job waits
   auto p = std::make_unique<Pool>();
   p->L = L;
@@ -163,6 +194,7 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   rc = rc ? rc : p->alloc(&p->bin_state, (size_t)cap * 2);
   rc = rc ? rc : p->alloc(&p->bins, (size_t)cap * 13 * kBinCap);
   rc = rc ? rc : p->alloc(&p->group.d_index, (size_t)cap);
+  rc = rc ? rc : p->alloc(&p->corr_planes, (size_t)corr_plane_doubles(L));
   if (!p->resident) {
     rc = rc ? rc : p->alloc(&g.bondmask, n);
     rc = rc ? rc : p->alloc(&g.visited, n / 32);
@@ -176,6 +208,7 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   p->group.L = L;
   p->group.cpt = 1;
   p->slot_job.assign(cap, -1);
+  p->done.assign(cap, 0);
   p->host.assign(cap, RepDev{});
   s->used += p->bytes;
   out = std::move(p);
@@ -247,8 +280,71 @@ int admit(slsf_server* s, Pool& p, int slot, Job& j) {
   // the host vectors above must outlive the async copies
   SRV_TRY(hipStreamSynchronize(p.stream));
   p.slot_job[slot] = j.id;
+  p.done[slot] = 0;
   j.slot = slot;
   j.start_s = now_since(s->t0);
+  if (j.corr()) {
+    j.next_sample = next_sample_after(j, 0);
+    j.bin_sum.assign(j.nseries(), 0.0);
+  }
+  return SLSF_OK;
+}
+
+// Sweeps of the pool's next chunk: the server's chunk, ended early at the
+// next correlation sample of any active slot.
+int64_t chunk_sweeps(slsf_server* s, const Pool& p) {
+  int64_t n = s->chunk;
+  for (int slot : p.active_idx) {
+    auto it = s->running.find(p.slot_job[slot]);
+    if (it == s->running.end()) continue;
+    const Job& j = it->second;
+    if (j.corr() && j.next_sample > 0) n = std::min(n, j.next_sample - p.done[slot]);
+  }
+  return std::max<int64_t>(n, 1);
+}
+
+// One correlation sample of the job in `slot` (its angles are at the sample's
+// sweep): plane sums on the device, the fold of slsf_measure_correlations on
+// the host, added to the job's open bin.
+int sample_correlations(Pool& p, int slot, Job& j) {
+  const int L = p.L, rxy = j.rxy, rz = j.rz;
+  if (p.cs_stale) {  // slab chunks left the trig cache behind the angles
+    launch_refresh<float>(p.g, p.stream);
+    p.cs_stale = false;
+  }
+  launch_corr_planes<float>(p.g, slot, L, L, rxy, rz, p.corr_planes, p.stream);
+  SRV_TRY(hipGetLastError());
+  const size_t n = (size_t)2 * rxy * L + rz;
+  p.corr_host.resize(std::max<size_t>(n, 1));
+  SRV_TRY(hipMemcpyAsync(p.corr_host.data(), p.corr_planes, sizeof(double) * n,
+                         hipMemcpyDeviceToHost, p.stream));
+  SRV_TRY(hipStreamSynchronize(p.stream));
+  const double* px = p.corr_host.data();
+  const double* py = px + (size_t)rxy * L;
+  const double* pz = py + (size_t)rxy * L;
+  const double vol = (double)L * L * L, area = (double)L * L;
+  double* sum = j.bin_sum.data();
+  // series order: CorrX, CorrY, CorrXY, CorrXY_z (layer-major), CorrZ
+  for (int rr = 0; rr < rxy; ++rr) {
+    double sx = 0.0, sy = 0.0;
+    for (int z = 0; z < L; ++z) {
+      const double lsx = px[(size_t)rr * L + z], lsy = py[(size_t)rr * L + z];
+      sx += lsx;
+      sy += lsy;
+      sum[3 * rxy + (size_t)z * rxy + rr] += (lsx + lsy) / (2.0 * area);
+    }
+    sum[rr] += sx / vol;
+    sum[rxy + rr] += sy / vol;
+    sum[2 * rxy + rr] += (sx / vol + sy / vol) / 2.0;
+  }
+  for (int rr = 0; rr < rz; ++rr) sum[3 * rxy + (size_t)L * rxy + rr] += pz[rr] / vol;
+  if (++j.in_bin == j.binsize) {
+    for (double& v : j.bin_sum) {
+      j.bin_means.push_back(v / j.binsize);
+      v = 0.0;
+    }
+    j.in_bin = 0;
+  }
   return SLSF_OK;
 }
 
@@ -258,15 +354,16 @@ int run_chunk(slsf_server* s, Pool& p) {
     if (p.slot_job[i] >= 0) p.active_idx.push_back(i);
   p.running = !p.active_idx.empty();
   if (!p.running) return SLSF_OK;
+  const int64_t n = chunk_sweeps(s, p);
   if (p.resident) {
     p.group.replicas = p.active_idx;
     SRV_TRY(hipMemcpyAsync(p.group.d_index, p.active_idx.data(),
                            sizeof(int32_t) * p.active_idx.size(), hipMemcpyHostToDevice,
                            p.stream));
     ResidentRunArgs ra;
-    ra.thermalization = s->chunk;  // each slot's window cuts its own tail
+    ra.thermalization = n;  // each slot's window cuts its own tail
     ra.sweeps = 0;
-    ra.sweeps_per_launch = s->chunk;
+    ra.sweeps_per_launch = n;
     ra.accum = p.accum;
     ra.bins = p.bins;
     ra.bin_state = p.bin_state;
@@ -289,7 +386,7 @@ int run_chunk(slsf_server* s, Pool& p) {
     bt.bins = p.bins;
     bt.bin_state = p.bin_state;
     bt.bin_cap = kBinCap;
-    (void)enqueue_hbm_sweeps<float>(g, s->chunk, wmax, bt, p.stream);
+    (void)enqueue_hbm_sweeps<float>(g, n, wmax, bt, p.stream);
   }
   SRV_TRY(hipGetLastError());
   return SLSF_OK;
@@ -305,6 +402,16 @@ int retire(slsf_server* s, Pool& p, int32_t* nfinished) {
                     hipMemcpyDeviceToHost));
   for (int i = 0; i < p.cap; ++i)
     for (int k = 0; k < TALLY_SLOTS; ++k) acc[i] += slots[(size_t)i * TALLY_SLOTS + k];
+  for (int slot : p.active_idx) {
+    p.done[slot] = devs[slot].metro_ctr;
+    auto it = s->running.find(p.slot_job[slot]);
+    if (it == s->running.end() || !it->second.corr()) continue;
+    Job& j = it->second;
+    if (j.next_sample > 0 && p.done[slot] == j.next_sample) {
+      if (int rc = sample_correlations(p, slot, j)) return rc;
+      j.next_sample = next_sample_after(j, p.done[slot]);
+    }
+  }
   const double t = now_since(s->t0);
   for (int slot : p.active_idx) {
     const RepDev& d = devs[slot];
@@ -338,6 +445,18 @@ int retire(slsf_server* s, Pool& p, int32_t* nfinished) {
         for (int b = 0; b < nb; ++b) m += bins[(size_t)k * nb + b];
         r.series_mean[k] = m / nb;
       }
+    }
+    if (j.corr()) {
+      const int ns = j.nseries(), cb = (int)(j.bin_means.size() / (size_t)ns);
+      r.corr_rxy = j.rxy;
+      r.corr_rz = j.rz;
+      r.corr_nbins = cb;
+      r.corr_nseries = ns;
+      std::vector<double> series((size_t)ns * cb);
+      for (int b = 0; b < cb; ++b)
+        for (int k = 0; k < ns; ++k)
+          series[(size_t)k * cb + b] = j.bin_means[(size_t)b * ns + k];
+      s->corr_done[j.id] = std::move(series);
     }
     s->site_updates += r.site_updates;
     s->done.push_back(r);
@@ -405,6 +524,8 @@ int slsf_server_submit(slsf_server* s, const slsf_job_request* q, int64_t* job_i
     return fail(SLSF_E_INVALID, "a job must have fewer than 2^32 sweeps");
   if ((int64_t)q->l * q->l * q->l >= (int64_t)1 << 31)
     return fail(SLSF_E_INVALID, "lattice volume must be below 2^31");
+  if (q->correlation_rmax_xy < 0 || q->correlation_rmax_z < 0)
+    return fail(SLSF_E_INVALID, "correlation radii must be nonnegative");
   if (slot_bytes(q->l) > s->budget)
     return fail(SLSF_E_ADMISSION, "a slot of L=" + std::to_string(q->l) + " needs " +
                                       std::to_string(slot_bytes(q->l)) +
@@ -414,6 +535,10 @@ int slsf_server_submit(slsf_server* s, const slsf_job_request* q, int64_t* job_i
   j.id = s->next_id++;
   j.req = *q;
   j.binsize = std::max<int32_t>(q->binsize, (int32_t)((q->sweeps + kBinCap - 1) / kBinCap));
+  // src/model.rs:360-365
+  j.rxy = std::min(q->correlation_rmax_xy, q->l / 2);
+  j.rz = std::min(q->correlation_rmax_z, q->l / 2);
+  j.interval = std::max(1, q->correlation_interval);
   j.submit_s = now_since(s->t0);
   s->queue.push_back(j);
   if (job_id) *job_id = j.id;
@@ -464,6 +589,23 @@ int slsf_server_poll(slsf_server* s, slsf_job_result* out, int32_t cap, int32_t*
     s->done.pop_front();
   }
   *n = k;
+  return SLSF_OK;
+}
+
+int slsf_server_correlations(slsf_server* s, int64_t job_id, double* out, int64_t cap,
+                             int64_t* count) {
+  clear_error();
+  if (!s || !count) return fail(SLSF_E_INVALID, "null argument");
+  auto it = s->corr_done.find(job_id);
+  if (it == s->corr_done.end())
+    return fail(SLSF_E_INVALID, "job " + std::to_string(job_id) +
+                                    " has no correlation series to take (not finished, "
+                                    "not measuring correlations, or taken already)");
+  *count = (int64_t)it->second.size();
+  if (!out) return SLSF_OK;
+  if (cap < *count) return fail(SLSF_E_INVALID, "output buffer too small");
+  std::copy(it->second.begin(), it->second.end(), out);
+  s->corr_done.erase(it);
   return SLSF_OK;
 }
 

@@ -36,6 +36,15 @@ SERIES = ["Energy", "EnergySquared", "MagnetizationSquared", "MagnetizationFourt
 EVALS = ["Magnetization", "Chi", "SpecificHeat", "RhoXY", "RhoZ", "Binder", "Energy"]
 
 
+def correlation_series_names(rxy: int, rz: int, lz: int) -> list:
+    """Correlation series names in the order src/model.rs:413-435 measures them."""
+    r = range(1, rxy + 1)
+    return ([f"CorrX_r{k}" for k in r] + [f"CorrY_r{k}" for k in r] +
+            [f"CorrXY_r{k}" for k in r] +
+            [f"CorrXY_z{z}_r{k}" for z in range(lz) for k in r] +
+            [f"CorrZ_r{k}" for k in range(1, rz + 1)])
+
+
 class SlsfError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(message)
@@ -85,7 +94,8 @@ class JobRequest(ctypes.Structure):
                 ("j_perp", ctypes.c_double), ("disorder_seed", ctypes.c_uint64),
                 ("seed", ctypes.c_uint64), ("thermalization", ctypes.c_int64),
                 ("sweeps", ctypes.c_int64), ("binsize", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("correlation_interval", ctypes.c_int32),
+                ("correlation_rmax_xy", ctypes.c_int32), ("correlation_rmax_z", ctypes.c_int32)]
 
 
 class JobResult(ctypes.Structure):
@@ -93,7 +103,9 @@ class JobResult(ctypes.Structure):
                 ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double),
                 ("finish_s", ctypes.c_double), ("acceptance", ctypes.c_double),
                 ("value", ctypes.c_double * 7), ("error", ctypes.c_double * 7),
-                ("series_mean", ctypes.c_double * 13), ("site_updates", ctypes.c_int64)]
+                ("series_mean", ctypes.c_double * 13), ("site_updates", ctypes.c_int64),
+                ("corr_rxy", ctypes.c_int32), ("corr_rz", ctypes.c_int32),
+                ("corr_nbins", ctypes.c_int32), ("corr_nseries", ctypes.c_int32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -209,6 +221,8 @@ _SIGS = {
     "slsf_server_step": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "slsf_server_poll": (ctypes.c_int, [ctypes.c_void_p, _P(JobResult), ctypes.c_int32, _i32p]),
     "slsf_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, _P(ServerStats)]),
+    "slsf_server_correlations": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _dp,
+                                                ctypes.c_int64, _i64p]),
     "slsf_server_slot_bytes": (ctypes.c_int64, [ctypes.c_int32]),
     "slsf_device_memory": (ctypes.c_int, [_i64p, _i64p]),
     "slsf_measure_ceilings": (ctypes.c_int, [ctypes.c_void_p]),
@@ -599,10 +613,14 @@ class Server:
 
     def submit(self, l, temperature, seed, disorder_seed, sweeps, thermalization=0,
                binsize=10, j_xy_min=0.6, j_xy_max=1.8, j_perp=1.0,
-               proposal_width=float(np.pi), wolff_steps=1) -> int:
+               proposal_width=float(np.pi), wolff_steps=1, correlation_interval=1,
+               correlation_rmax_xy=0, correlation_rmax_z=0) -> int:
+        """Queues a job; with a correlation radius above 0 its result carries
+        the binned correlation series under "correlations"."""
         q = JobRequest(l, wolff_steps, temperature, proposal_width, j_xy_min, j_xy_max, j_perp,
                        disorder_seed & 0xFFFFFFFFFFFFFFFF, seed & 0xFFFFFFFFFFFFFFFF,
-                       thermalization, sweeps, binsize, 0)
+                       thermalization, sweeps, binsize, correlation_interval,
+                       correlation_rmax_xy, correlation_rmax_z)
         jid = ctypes.c_int64()
         _check(_lib.slsf_server_submit(self._h, ctypes.byref(q), ctypes.byref(jid)))
         return jid.value
@@ -623,9 +641,22 @@ class Server:
                 d["value"] = dict(zip(EVALS, list(r.value)))
                 d["error"] = dict(zip(EVALS, list(r.error)))
                 d["series_mean"] = dict(zip(SERIES, list(r.series_mean)))
+                if r.corr_nseries > 0:
+                    d["correlations"] = self._take_correlations(r)
                 out.append(d)
             if n.value < 256:
                 return out
+
+    def _take_correlations(self, r) -> dict:
+        """{series name: bin means} of a finished job (slsf_server_correlations)."""
+        count = ctypes.c_int64()
+        _check(_lib.slsf_server_correlations(self._h, r.job_id, None, 0, ctypes.byref(count)))
+        v = np.zeros(max(count.value, 1))
+        _check(_lib.slsf_server_correlations(self._h, r.job_id, _ptr(v, ctypes.c_double),
+                                             count.value, ctypes.byref(count)))
+        v = v[:count.value].reshape(r.corr_nseries, r.corr_nbins)
+        names = correlation_series_names(r.corr_rxy, r.corr_rz, r.l)
+        return {name: v[k].copy() for k, name in enumerate(names)}
 
     def stats(self) -> dict:
         st = ServerStats()

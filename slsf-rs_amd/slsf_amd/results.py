@@ -50,7 +50,8 @@ from typing import Sequence
 
 import numpy as np
 
-from . import (EVALS, FP32, SERIES, Batch, Job, Replica, SlsfError, evaluate)
+from . import (EVALS, FP32, SERIES, Batch, Job, Replica, SlsfError, correlation_series_names,
+               evaluate)
 from . import hdf5
 
 # fields of ThetaTask (src/model.rs:20-47), in declaration order
@@ -104,13 +105,7 @@ def _corr_spec(t: dict):
     return rxy, rz, max(1, int(t["correlation_interval"]))
 
 
-def _corr_names(rxy: int, rz: int, lz: int) -> list:
-    """Series names in the order src/model.rs:419-435 measures them."""
-    r = range(1, rxy + 1)
-    return ([f"CorrX_r{k}" for k in r] + [f"CorrY_r{k}" for k in r] +
-            [f"CorrXY_r{k}" for k in r] +
-            [f"CorrXY_z{z}_r{k}" for z in range(lz) for k in r] +
-            [f"CorrZ_r{k}" for k in range(1, rz + 1)])
+_corr_names = correlation_series_names
 
 
 class _Timers:

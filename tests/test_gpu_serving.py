@@ -7,6 +7,8 @@
   on the device; a device-memory budget that fits one slot serialises jobs;
   a job whose slot can never fit is refused with a clear message.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -143,3 +145,80 @@ def test_served_job_equals_batch_run():
             for v in bins[k]:
                 m += v
             assert r["series_mean"][name] == m / bins.shape[1], (L, name)
+
+
+def _batch_correlation_bins(a, rxy, rz):
+    """Bin means of the correlation samples of a served job's chain, run
+    through Batch.run in segments that end on the sampled sweeps (the
+    measurements m with m % interval == 0, src/model.rs:410-412)."""
+    import oracle_lib as orc
+    L, c = a["l"], max(1, a["correlation_interval"])
+    jxy = orc.uniform_layers(L, 0.6, 1.8, a["disorder_seed"])
+    b = slsf.Batch([slsf.Replica(L, L, L, a["temperature"], jxy, np.ones(L), a["seed"])],
+                   slsf.FP32)
+    b.init_angles()
+    if a["thermalization"]:
+        b.run(a["thermalization"], 0, 1)
+    samples, done = [], 0
+    for m in range(0, a["sweeps"], c):
+        b.run(0, m + 1 - done, 1)
+        done = m + 1
+        o = b.correlations(0, -1, rxy, rz)
+        samples.append(np.concatenate([o["corr_x"], o["corr_y"], o["corr_xy"],
+                                       o["corr_xy_by_z"].ravel(), o["corr_z"]]))
+    b.close()
+    bs = a["binsize"]
+    nb = len(samples) // bs
+    out = np.zeros((nb, len(samples[0])))
+    for i in range(nb):
+        acc = np.zeros(len(samples[0]))
+        for v in samples[i * bs:(i + 1) * bs]:
+            acc = acc + v
+        out[i] = acc / bs
+    return out
+
+
+def test_served_correlation_series_equal_batch_samples():
+    """Correlation series of served jobs (src/model.rs:410-435): the server
+    ends chunks on each job's sampled sweeps, so a resident-pool job (L=16,
+    every third measurement, radii clamped to L/2) and an HBM-pool job (L=48,
+    every second measurement, in-plane only) give the bin means of the
+    samples Batch.correlations takes on the same chains, bit for bit (the
+    plane sums add the same row dots in the same order).  A job without
+    correlations in the same pool keeps its result (the shorter chunks do
+    not change its chain), and its result carries no correlation series."""
+    with_corr = [dict(job_args(16, 900, sweeps=40, therm=10, T=1.1, binsize=4),
+                      correlation_interval=3, correlation_rmax_xy=20, correlation_rmax_z=5),
+                 dict(job_args(48, 901, sweeps=30, therm=7, T=1.4, binsize=3),
+                      correlation_interval=2, correlation_rmax_xy=3, correlation_rmax_z=0)]
+    plain = job_args(16, 902, sweeps=60, therm=5, T=1.3)
+    alone = slsf.Server(8 * GB, 4, chunk_sweeps=25)
+    alone.submit(**plain)
+    plain_alone = alone.run_until_idle()[0]
+    assert "correlations" not in plain_alone
+    srv = slsf.Server(8 * GB, 4, chunk_sweeps=25)
+    ids = [srv.submit(**a) for a in with_corr]
+    pid = srv.submit(**plain)
+    res = {r["job_id"]: r for r in srv.run_until_idle()}
+    same_result(plain_alone, res[pid])
+    assert "correlations" not in res[pid]
+    for jid, a in zip(ids, with_corr):
+        L = a["l"]
+        rxy, rz = min(a["correlation_rmax_xy"], L // 2), min(a["correlation_rmax_z"], L // 2)
+        r = res[jid]
+        assert (r["corr_rxy"], r["corr_rz"]) == (rxy, rz)
+        expect = _batch_correlation_bins(a, rxy, rz)
+        names = slsf.correlation_series_names(rxy, rz, L)
+        assert sorted(r["correlations"]) == sorted(names)
+        assert r["corr_nbins"] == expect.shape[0] > 0
+        got = np.stack([r["correlations"][n] for n in names], axis=1)
+        exact = int(np.sum(got == expect))
+        print(f"L={L}: {got.size} bin means, {exact} bit-identical, max |diff| "
+              f"{np.max(np.abs(got - expect)):.2e}")
+        # same chain, same dots, same fold order: bit for bit
+        assert exact == got.size, L
+    # the series were taken by poll: a second take is refused
+    cnt = ctypes.c_int64()
+    with pytest.raises(slsf.SlsfError, match="no correlation series"):
+        slsf._check(slsf.lib().slsf_server_correlations(srv._h, ids[0], None, 0,
+                                                        ctypes.byref(cnt)))
