@@ -290,6 +290,53 @@ def instruction_record(nrep):
     return None
 
 
+def _wrapped_diff(a, b):
+    d = np.abs(a - b) % (2 * np.pi)
+    return np.minimum(d, 2 * np.pi - d)
+
+
+def fp32_accuracy(slsf, n_metro=32, n_full=8):
+    """FP32 production arithmetic against FP64 on the same uniforms: one sweep
+    from the same keyed initial angles by the FP32 resident kernel (its
+    replay-compatible stream) and by the FP64 replay kernels, over the headline
+    batch's first realizations (all 8 temperatures).  Metropolis only: a site
+    whose angles differ by more than 1e-3 rad took the other decision; max
+    |dtheta| is over the agreeing sites.  Metropolis + Wolff: replicas whose
+    sweep differs anywhere by more than the FP32 rounding of an angle."""
+    out = {}
+    for key, n_real, wolff in (("metropolis", n_metro, 0), ("metropolis_wolff", n_full, 1)):
+        reps = c3_replicas(slsf, n_real, wolff_steps=wolff)
+        b64 = slsf.Batch(reps, slsf.FP64_REPLAY)
+        b32 = slsf.Batch(reps, slsf.FP32)
+        b64.init_angles()
+        b32.init_angles()
+        b64.sweep(1)
+        b32.run(1, 0, 1, replay_stream=True)
+        bad, maxd, differ = 0, 0.0, 0
+        for r in range(len(reps)):
+            d = _wrapped_diff(b64.get_theta(r), b32.get_theta(r))
+            mism = d > 1e-3
+            bad += int(mism.sum())
+            if (~mism).any():
+                maxd = max(maxd, float(d[~mism].max()))
+            differ += int(mism.any())
+        b64.close()
+        b32.close()
+        updates = len(reps) * 32 ** 3
+        if wolff:
+            out[key] = {"replicas": len(reps), "replicas_differing": differ,
+                        "max_abs_dtheta_rad": maxd}
+        else:
+            out[key] = {"replicas": len(reps), "site_updates": updates,
+                        "decision_mismatches": bad,
+                        "mismatches_per_million": bad / updates * 1e6,
+                        "max_abs_dtheta_rad": maxd}
+    out["run"] = (f"one sweep from the keyed initial angles, L=32, realizations 0..{n_metro - 1} "
+                  f"(Metropolis only) and 0..{n_full - 1} (with the Wolff step) x 8 T of the "
+                  "batched workload; FP32 resident kernel on the replay stream vs FP64 replay")
+    return out
+
+
 # ---------------------------------------------------------------------------
 def bench_c3(args, slsf):
     reps = c3_replicas(slsf, args.realizations)
@@ -372,6 +419,7 @@ def bench_c3(args, slsf):
     hbm_equiv = {"bytes_per_update": 8.0, "achieved_GBs": kernel_rate * 8.0 / 1e9,
                  "peak": HBM_PEAK_GBS, "frac": kernel_rate * 8.0 / 1e9 / HBM_PEAK_GBS}
 
+    accuracy = fp32_accuracy(slsf)
     cpu, agreement, agreement_note = None, None, None
     if not args.no_cpu_baseline:
         plan = cpu_sample_plan()
@@ -404,7 +452,7 @@ def bench_c3(args, slsf):
          metropolis_only_rate=metro_rate, wolff_only_rate=wolff_rate,
          acceptance=st["acceptance_mean"], mean_cluster_size=st["mean_cluster_size"],
          roofline=roofline, valu_issue=valu, hbm_equivalent=hbm_equiv, ceilings=ceil,
-         cpu_baseline=cpu, fp64_cpu_agreement=agreement,
+         cpu_baseline=cpu, fp32_same_uniforms=accuracy, fp64_cpu_agreement=agreement,
          fp64_cpu_agreement_run=agreement_note,
          kernel={"name": "k_resident<32, 1, paired>", "launches": st["launches"],
                  "avg_launch_ms": avg_launch_ms, "sweeps_per_launch": spl,
@@ -467,6 +515,13 @@ def bench_c4(args, slsf):
     st = b.run(0, sweeps, binsize)
     updates = sum(r.volume for r in reps) * (therm + sweeps)
     value = sum(r.volume for r in reps) * sweeps / st["seconds"]
+    # the same batch on the library's default plan (no autotuning), same sweeps
+    u = slsf.Batch(reps, slsf.FP32)
+    u.init_angles()
+    u.run(therm, 0, 1)
+    su = u.run(0, sweeps, binsize)
+    u.close()
+    untuned = sum(r.volume for r in reps) * sweeps / su["seconds"]
     # disorder averages per (L, T) with jackknife errors over realizations
     # (slsf_disorder_average: Binder = [<m^4>]/[<m^2>]^2, helicity moduli
     # averaged over realizations), and the crossings of the Binder ratio and of
@@ -495,7 +550,8 @@ def bench_c4(args, slsf):
          {"workload": "uneven batch, autotuned: L in {8,12,16,24,32} x 5 T in [1.1,1.5] x 64 "
                       "realizations (1600 replicas), 1000 + 4000 sweeps, binsize 100",
           "replicas": len(reps)},
-         "fp32 (fp64 accumulators)", autotune=tuned, binder_rho=table,
+         "fp32 (fp64 accumulators)", autotune=tuned, untuned_rate=untuned,
+         tuned_over_untuned=value / untuned, binder_rho=table,
          crossings=crossings, acceptance=st["acceptance_mean"],
          mean_cluster_size=st["mean_cluster_size"], total_site_updates=updates)
 
