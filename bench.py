@@ -482,24 +482,30 @@ def bench_c1(args, slsf):
 
 
 def bench_c2(args, slsf):
-    L, T = 64, 1.3
-    rep = slsf.Replica(L, L, L, T, splitmix_uniform_layers(L, 1), np.ones(L),
-                       task_seed(1, L, 1, T))
+    """Replicas at scale on the HBM kernels: one L=64 replica by default;
+    --c2-l / --c2-replicas size it otherwise (realization r has the couplings
+    of seed 1 + r), --c2-fp32-only leaves out the FP64 replay run."""
+    L, T, n, sweeps = args.c2_l, 1.3, args.c2_replicas, args.c2_sweeps
+    reps = [slsf.Replica(L, L, L, T, splitmix_uniform_layers(L, 1 + r), np.ones(L),
+                         task_seed(1, L, 1 + r, T)) for r in range(n)]
     res = {}
-    for name, prec in (("fp64_replay", slsf.FP64_REPLAY), ("fp32", slsf.FP32)):
-        b = slsf.Batch([rep], prec)
+    precs = (("fp32", slsf.FP32),) if args.c2_fp32_only else \
+        (("fp64_replay", slsf.FP64_REPLAY), ("fp32", slsf.FP32))
+    for name, prec in precs:
+        b = slsf.Batch(reps, prec)
         b.init_angles()
         b.run(20, 0, 1)
-        st = b.run(0, 1000, 10)
+        st = b.run(0, sweeps, 10)
         b.close()
-        res[name] = {"rate": L ** 3 * 1000 / st["seconds"], "seconds": st["seconds"],
+        res[name] = {"rate": n * L ** 3 * sweeps / st["seconds"], "seconds": st["seconds"],
                      "acceptance": st["acceptance_mean"],
                      "mean_cluster_size": st["mean_cluster_size"]}
     f = res["fp32"]
-    line(f["rate"], "site-updates/s", 1000, 20, 1e3 * f["seconds"] / 1000,
-         {"workload": "single replica at scale: L=64, T=1.3, 1000 Metropolis + Wolff sweeps "
-                      "with measurement, HBM kernels"},
-         "fp32 (fp64 accumulators)", fp64_replay=res["fp64_replay"], fp32=f)
+    what = "single replica" if n == 1 else f"{n} replicas"
+    line(f["rate"], "site-updates/s", sweeps, 20, 1e3 * f["seconds"] / sweeps,
+         {"workload": f"{what} at scale: L={L}, T=1.3, {sweeps} Metropolis + Wolff sweeps "
+                      "with measurement, HBM kernels", "L": L, "replicas": n},
+         "fp32 (fp64 accumulators)", fp64_replay=res.get("fp64_replay"), fp32=f)
 
 
 def bench_c4(args, slsf):
@@ -621,6 +627,10 @@ def main():
     ap.add_argument("--realizations", type=int, default=256)
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--serve-window", type=float, default=3.0)
+    ap.add_argument("--c2-l", type=int, default=64)
+    ap.add_argument("--c2-replicas", type=int, default=1)
+    ap.add_argument("--c2-sweeps", type=int, default=1000)
+    ap.add_argument("--c2-fp32-only", action="store_true")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c3")
     ap.add_argument("--mode", choices=["bench", "copy"], default="bench",
                     help="copy: a plain 1 GiB device copy, to calibrate HBM byte counters")
