@@ -2,12 +2,18 @@
 
 * A job's result is bit-identical alone and in a mix of jobs of other sizes,
   temperatures and lengths, even with a different chunk size (keyed uniforms,
-  per-replica windows and reductions).
+  per-replica windows and reductions), as long as its pool keeps one form of
+  the HBM path; where the number of active jobs moves an HBM pool across the
+  slab / per-phase crossover, the chain is the same and the bins differ in
+  the order of their FP64 sums only.
+* Served correlation series equal Batch's samples bit for bit, on resident,
+  slab and per-phase pools.
 * Jobs that arrive while others run are admitted as slots free up and overlap
   on the device; a device-memory budget that fits one slot serialises jobs;
   a job whose slot can never fit is refused with a clear message.
 """
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -222,3 +228,36 @@ def test_served_correlation_series_equal_batch_samples():
     with pytest.raises(slsf.SlsfError, match="no correlation series"):
         slsf._check(slsf.lib().slsf_server_correlations(srv._h, ids[0], None, 0,
                                                         ctypes.byref(cnt)))
+
+
+def test_served_correlations_on_a_per_phase_pool():
+    """An L=48 pool with twenty active jobs (540 16-deep tiles, above the
+    slab form's 512) runs the per-phase kernels, which keep the trig cache
+    current themselves: a correlation job among them (z and in-plane radii,
+    every measurement sampled) still gives the bin means of Batch's samples on
+    the same chain, bit for bit, and its neighbours keep their results."""
+    corr = dict(job_args(48, 950, sweeps=12, therm=3, T=1.2, binsize=3),
+                correlation_interval=1, correlation_rmax_xy=2, correlation_rmax_z=3)
+    others = [job_args(48, 960 + i, sweeps=12 + i % 3, therm=3, T=1.0 + 0.05 * i)
+              for i in range(19)]
+    srv = slsf.Server(8 * GB, 32, chunk_sweeps=5)
+    cid = srv.submit(**corr)
+    oids = [srv.submit(**a) for a in others]
+    res = {r["job_id"]: r for r in srv.run_until_idle()}
+    assert len(res) == 20
+    r = res[cid]
+    expect = _batch_correlation_bins(corr, 2, 3)
+    names = slsf.correlation_series_names(2, 3, 48)
+    got = np.stack([r["correlations"][n] for n in names], axis=1)
+    assert r["corr_nbins"] == expect.shape[0] == 4
+    assert np.array_equal(got, expect)
+    # one neighbour against its chain run alone, where its pool takes the slab
+    # form: the same chain (acceptance counts equal), bins summed in the other
+    # form's order (equal to FP64 rounding)
+    alone = slsf.Server(8 * GB, 4, chunk_sweeps=50)
+    alone.submit(**others[7])
+    a, b = alone.run_until_idle()[0], res[oids[7]]
+    assert a["nbins"] == b["nbins"] and a["acceptance"] == b["acceptance"]
+    for k in a["series_mean"]:
+        assert math.isclose(a["series_mean"][k], b["series_mean"][k], rel_tol=1e-12,
+                            abs_tol=1e-14), k
