@@ -436,7 +436,8 @@ int slsf_server_get_stats(const slsf_server* server, slsf_server_stats* out);
  * job's series until this call takes them. */
 int slsf_server_correlations(slsf_server* server, int64_t job_id, double* out, int64_t cap,
                              int64_t* count);
-/* Bytes one slot of size l occupies (the admission footprint). */
+/* Bytes one slot of size l occupies (the admission footprint); a pool of size
+ * l adds (l^2 + l) doubles for the correlation sums of its samples. */
 int64_t slsf_server_slot_bytes(int32_t l);
 /* Free and total memory of the current device (hipMemGetInfo), for choosing a
  * server budget. */

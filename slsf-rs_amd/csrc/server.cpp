@@ -76,6 +76,10 @@ struct SplitMix64 {
 
 // plane sums of one correlation sample: 2 rxy l + rz doubles with rxy, rz <= l/2
 int64_t corr_plane_doubles(int L) { return (int64_t)L * L + L; }
+// device bytes a pool of size L holds besides its slots (the plane sums)
+int64_t pool_fixed_bytes(int L) {
+  return std::max<int64_t>(corr_plane_doubles(L) * (int64_t)sizeof(double), 256);
+}
 
 struct Job {
   int64_t id;
@@ -163,8 +167,7 @@ namespace {
 
 int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
   const int64_t per = slot_bytes(L);
-  const int64_t planes = corr_plane_doubles(L) * (int64_t)sizeof(double);
-  const int64_t room = s->budget - s->used - std::max<int64_t>(planes, 256);
+  const int64_t room = s->budget - s->used - pool_fixed_bytes(L);
   const int cap = (int)std::min<int64_t>(s->max_slots, std::max<int64_t>(room, 0) / per);
   if (cap <= 0) return SLSF_OK;  // not now: the job waits
   auto p = std::make_unique<Pool>();
@@ -526,9 +529,9 @@ int slsf_server_submit(slsf_server* s, const slsf_job_request* q, int64_t* job_i
     return fail(SLSF_E_INVALID, "lattice volume must be below 2^31");
   if (q->correlation_rmax_xy < 0 || q->correlation_rmax_z < 0)
     return fail(SLSF_E_INVALID, "correlation radii must be nonnegative");
-  if (slot_bytes(q->l) > s->budget)
-    return fail(SLSF_E_ADMISSION, "a slot of L=" + std::to_string(q->l) + " needs " +
-                                      std::to_string(slot_bytes(q->l)) +
+  if (slot_bytes(q->l) + pool_fixed_bytes(q->l) > s->budget)
+    return fail(SLSF_E_ADMISSION, "a pool of one L=" + std::to_string(q->l) + " slot needs " +
+                                      std::to_string(slot_bytes(q->l) + pool_fixed_bytes(q->l)) +
                                       " bytes, more than the device budget of " +
                                       std::to_string(s->budget));
   Job j;

@@ -99,6 +99,13 @@ def test_memory_budget_admission():
     with pytest.raises(slsf.SlsfError) as e:
         srv.submit(**job_args(64, 3))
     assert e.value.code == 3 and "more than the device budget" in e.value.message
+    # a budget that holds one slot but not its pool's correlation sums refuses
+    # the job at once instead of queueing it for ever
+    tight = slsf.Server(one + 100, 8, chunk_sweeps=50)
+    with pytest.raises(slsf.SlsfError) as e:
+        tight.submit(**job_args(16, 5, sweeps=100))
+    assert e.value.code == 3 and "more than the device budget" in e.value.message
+    tight.close()
     with pytest.raises(slsf.SlsfError) as e:
         srv.submit(**dict(job_args(16, 4), temperature=0.0))
     assert e.value.message == "temperature T must be positive and finite"
