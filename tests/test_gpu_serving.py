@@ -268,3 +268,38 @@ def test_served_correlations_on_a_per_phase_pool():
     for k in a["series_mean"]:
         assert math.isclose(a["series_mean"][k], b["series_mean"][k], rel_tol=1e-12,
                             abs_tol=1e-14), k
+
+
+def test_correlation_requests_and_takes_are_checked():
+    """Negative radii are refused at submit; a take into a buffer that is too
+    small is refused and leaves the series for a proper take."""
+    srv = slsf.Server(8 * GB, 4, chunk_sweeps=10)
+    with pytest.raises(slsf.SlsfError, match="correlation radii must be nonnegative"):
+        srv.submit(**dict(job_args(16, 970, sweeps=8), correlation_rmax_xy=-1))
+    a = dict(job_args(16, 971, sweeps=8, therm=0, binsize=2), correlation_interval=2,
+             correlation_rmax_xy=2, correlation_rmax_z=1)
+    jid = srv.submit(**a)
+    res = []
+    for _ in range(100):
+        srv.step()
+        n = ctypes.c_int32()
+        buf = (slsf.JobResult * 4)()
+        slsf._check(slsf.lib().slsf_server_poll(srv._h, buf, 4, ctypes.byref(n)))
+        res += list(buf[:n.value])
+        if res:
+            break
+    assert len(res) == 1 and res[0].job_id == jid and res[0].corr_nbins == 2
+    count = ctypes.c_int64()
+    slsf._check(slsf.lib().slsf_server_correlations(srv._h, jid, None, 0, ctypes.byref(count)))
+    assert count.value == res[0].corr_nseries * 2 == (3 * 2 + 16 * 2 + 1) * 2
+    small = np.zeros(count.value - 1)
+    with pytest.raises(slsf.SlsfError, match="output buffer too small"):
+        slsf._check(slsf.lib().slsf_server_correlations(
+            srv._h, jid, small.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(small),
+            ctypes.byref(count)))
+    full = np.zeros(count.value)
+    slsf._check(slsf.lib().slsf_server_correlations(
+        srv._h, jid, full.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(full),
+        ctypes.byref(count)))
+    assert np.all(np.isfinite(full)) and np.all(np.abs(full) <= 1.0 + 1e-12)
+    srv.close()

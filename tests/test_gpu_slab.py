@@ -193,3 +193,11 @@ def test_slab_and_per_phase_forms_give_the_same_chains_and_bins(precision):
             assert np.array_equal(th[i], ref[1][i]), (forms, i)
             assert np.allclose(bins[i], ref[2][i], rtol=1e-12, atol=1e-12), (forms, i)
         assert np.array_equal(ctr[0], ref[3][0]) and np.array_equal(ctr[1], ref[3][1])
+
+
+def test_hbm_form_is_checked():
+    b = slsf.Batch([_pair(48, 48, 48, 1.3, 1, 1)[0]], slsf.FP32)
+    b.init_angles()
+    with pytest.raises(slsf.SlsfError, match="hbm_form must be 0"):
+        b.run(1, 0, 1, hbm_form=3)
+    b.close()
