@@ -100,7 +100,7 @@ int ensure_slab_buffers(slsf_batch* b) {
   if (b->g.theta2) return SLSF_OK;
   const size_t rs = is_fp64(b) ? sizeof(double) : sizeof(float);
   if (int rc = b->alloc((char**)&b->g.theta2, rs * (size_t)b->g.total_sites)) return rc;
-  return b->alloc(&b->g.slab_partials, (size_t)b->g.nrep * b->g.max_tiles8 * 9);
+  return b->alloc(&b->g.slab_partials, (size_t)b->g.nrep * b->g.max_tiles4 * 9);
 }
 
 int metro_phases(slsf_batch* b, uint8_t* d_decisions) {

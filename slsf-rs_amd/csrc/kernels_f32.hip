@@ -39,6 +39,7 @@ void set_extents(GlobalState& g, const RepDev* reps, int n) {
   g.obs_blocks = 0;
   g.max_tiles = 0;
   g.max_tiles8 = 0;
+  g.max_tiles4 = 0;
   for (int r = 0; r < n; ++r) {
     g.max_volume = std::max<int64_t>(g.max_volume, reps[r].volume);
     g.max_plane = std::max<int64_t>(g.max_plane, (int64_t)reps[r].lx * reps[r].ly);
@@ -46,6 +47,7 @@ void set_extents(GlobalState& g, const RepDev* reps, int n) {
     g.obs_blocks = std::max(g.obs_blocks, obs_blocks_of(reps[r].lx, reps[r].ly, reps[r].lz));
     g.max_tiles = std::max(g.max_tiles, wolff_tiles_of<16>(reps[r].lx, reps[r].ly, reps[r].lz));
     g.max_tiles8 = std::max(g.max_tiles8, wolff_tiles_of<8>(reps[r].lx, reps[r].ly, reps[r].lz));
+    g.max_tiles4 = std::max(g.max_tiles4, wolff_tiles_of<4>(reps[r].lx, reps[r].ly, reps[r].lz));
   }
 }
 
@@ -111,25 +113,25 @@ void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s
 
 // ---------------------------------------------------------------------------
 // per-measurement accumulation of the observable series into FP64 bins, on the
-// device: one wave per replica folds the block partials of k_observables (lane
-// k takes blocks k, k+64, ... in order, then a fixed shuffle tree), forms the
-// 13 series values and appends a bin mean every `binsize` measurements.  A
-// replica is measured iff it takes part in this sweep and its counter (not yet
-// advanced) is at least measure_from.  With rounds >= 0 the same block then does
-// the replica's k_sweep_end(metro = 1, rounds) bookkeeping, which saves a
-// launch on measured sweeps.
+// device: one workgroup per replica folds the block partials of k_observables
+// (fold_measurement's fixed order), forms the 13 series values and appends a
+// bin mean every `binsize` measurements.  A replica is measured iff it takes
+// part in this sweep and its counter (not yet advanced) is at least
+// measure_from.  With rounds >= 0 the same block then does the replica's
+// k_sweep_end(metro = 1, rounds) bookkeeping, which saves a launch on measured
+// sweeps.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(FOLD_THREADS)
 k_accumulate(RepDev* reps, const double* __restrict__ partials, int obs_stride, double* accum,
              double* bins, int32_t* bin_state, int32_t bin_cap, int rounds) {
   const int r = blockIdx.x;
   RepDev& d = reps[r];
-  const int lane = threadIdx.x;
   const bool measured = d.active && d.metro_ctr >= d.measure_from;
+  __syncthreads();  // every thread has read d before thread 0 advances it
   if (measured)
     fold_measurement(d, r, partials + (int64_t)r * obs_stride * 9,
-                     obs_blocks_of(d.lx, d.ly, d.lz), accum, bins, bin_state, bin_cap, lane);
-  if (rounds >= 0 && lane == 0) {  // as k_sweep_end(metro = 1, rounds), after every read of d
+                     obs_blocks_of(d.lx, d.ly, d.lz), accum, bins, bin_state, bin_cap);
+  if (rounds >= 0 && threadIdx.x == 0) {  // as k_sweep_end(metro = 1, rounds), after every read of d
     if (d.active) {
       d.metro_ctr += 1u;
       d.wolff_ctr += (uint32_t)(rounds < d.wolff_steps ? rounds : d.wolff_steps);
@@ -140,7 +142,7 @@ k_accumulate(RepDev* reps, const double* __restrict__ partials, int obs_stride, 
 
 void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
                        int32_t bin_cap, int rounds, hipStream_t s) {
-  k_accumulate<<<g.nrep, 64, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins, bin_state,
+  k_accumulate<<<g.nrep, FOLD_THREADS, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins, bin_state,
                                      bin_cap, rounds);
 }
 

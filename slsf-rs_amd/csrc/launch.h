@@ -22,6 +22,7 @@ struct GlobalState {
   int64_t max_plane = 0;
   int max_tiles = 0;        // largest wolff_tiles_of<16>
   int max_tiles8 = 0;       // largest wolff_tiles_of<8> (the shallow tiles of small launches)
+  int max_tiles4 = 0;       // largest wolff_tiles_of<4> (slab measurement quarters, smallest launches)
   uint8_t* bondmask = nullptr;        // [total_sites] active-bond bits of the base site
   uint32_t* visited = nullptr;        // [total_sites / 32] cluster membership bitmap
   uint32_t* label = nullptr;          // [total_sites] union-find forest of the cluster step
@@ -31,7 +32,7 @@ struct GlobalState {
   double* partials = nullptr;         // [nrep * obs_blocks * 9]
   int obs_blocks = 0;                 // max over replicas of obs_blocks_of
   // slab sweeps (enqueue_hbm_sweeps): the second angle buffer, R[total_sites],
-  // and the measurement partials of the tiles, [nrep * max_tiles8 * 9]
+  // and the measurement partials of the quarter tiles, [nrep * max_tiles4 * 9]
   void* theta2 = nullptr;
   double* slab_partials = nullptr;
 };

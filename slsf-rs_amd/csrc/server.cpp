@@ -51,14 +51,14 @@ int64_t padded_volume(int L) { return ((int64_t)L * L * L + 63) / 64 * 64; }
 // device bytes of one slot: angles + trig cache (12 B/site), couplings,
 // descriptor, accumulators and bins; the HBM path adds its Wolff scratch
 // (bond masks 1 B, union-find labels 4 B, membership bits), block partials,
-// and the slab sweeps' second angle buffer (4 B) and tile partials
+// and the slab sweeps' second angle buffer (4 B) and quarter-tile partials
 int64_t slot_bytes(int L) {
   const int64_t vp = padded_volume(L);
   int64_t b = vp * 12 + 2 * L * 4 + (int64_t)sizeof(RepDev) + 13 * 8 + 2 * 4 +
               13 * 8 * (int64_t)kBinCap + 2 * 8 * TALLY_SLOTS + 4;
   if (!resident_supports(L))
     b += vp * 5 + vp / 8 + (int64_t)obs_blocks_of(L, L, L) * 9 * 8 + (int64_t)sizeof(WolffDev) +
-         vp * 4 + (int64_t)((L + 15) / 16) * ((L + 15) / 16) * ((L + 7) / 8) * 9 * 8;
+         vp * 4 + (int64_t)((L + 15) / 16) * ((L + 15) / 16) * ((L + 3) / 4) * 9 * 8;
   return b;
 }
 
@@ -205,7 +205,7 @@ int create_pool(slsf_server* s, int L, std::unique_ptr<Pool>& out) {
     rc = rc ? rc : p->alloc(&g.wd, (size_t)cap);
     rc = rc ? rc : p->alloc(&g.partials, (size_t)cap * g.obs_blocks * 9);
     rc = rc ? rc : p->alloc((float**)&g.theta2, n);
-    rc = rc ? rc : p->alloc(&g.slab_partials, (size_t)cap * g.max_tiles8 * 9);
+    rc = rc ? rc : p->alloc(&g.slab_partials, (size_t)cap * g.max_tiles4 * 9);
   }
   if (rc) return rc;
   p->group.L = L;
