@@ -204,6 +204,14 @@ int ensure_trig_cache(slsf_batch* b) {
   return SLSF_OK;
 }
 
+// The per-phase kernels read and keep the (cos, sin) cache in FP64; in FP32
+// they form the pairs from the angles and leave the cache behind them, for
+// the readers of it (correlations, checkpoints) to refresh.
+int before_per_phase(slsf_batch* b) { return is_fp64(b) ? ensure_trig_cache(b) : SLSF_OK; }
+void after_per_phase(slsf_batch* b) {
+  if (!is_fp64(b)) b->cs_stale = true;
+}
+
 extern "C" {
 
 const char* slsf_last_error(void) { return g_error.c_str(); }
@@ -401,13 +409,14 @@ int slsf_metropolis_sweep(slsf_batch* b, double* acceptance, uint8_t* decisions)
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
-  if (int rc = ensure_trig_cache(b)) return rc;
+  if (int rc = before_per_phase(b)) return rc;
   if (decisions && !b->d_decisions)
     if (int rc = b->alloc(&b->d_decisions, (size_t)b->g.total_sites)) return rc;
   if (acceptance) std::fill(acceptance, acceptance + b->reps.size(), 0.0);
   launch_clear_counters(b->g, b->stream);
   launch_sweep_end(b->g, 0, 0, b->stream);
   if (int rc = metro_phases(b, decisions ? b->d_decisions : nullptr)) return rc;
+  after_per_phase(b);
   launch_sweep_end(b->g, 1, 0, b->stream);
   if (int rc = gather_acceptance(b, 1, acceptance)) return rc;
   if (decisions) {
@@ -428,7 +437,7 @@ int slsf_wolff_cluster_step(slsf_batch* b, int64_t* cluster_size, uint8_t* membe
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
-  if (int rc = ensure_trig_cache(b)) return rc;
+  if (int rc = before_per_phase(b)) return rc;
   // a standalone cluster step runs one cluster for every replica (round 0 with
   // wolff_steps forced >= 1 would change the descriptor; use a dedicated path)
   if (int rc = sync_reps_from_device(b)) return rc;
@@ -442,6 +451,7 @@ int slsf_wolff_cluster_step(slsf_batch* b, int64_t* cluster_size, uint8_t* membe
   launch_clear_counters(b->g, b->stream);
   launch_sweep_end(b->g, 0, 0, b->stream);
   int rc = wolff_round(b, 0, members);
+  after_per_phase(b);
   if (!rc) {
     launch_sweep_end(b->g, 0, 1, b->stream);
     rc = gather_clusters(b, std::vector<int64_t>(b->reps.size(), 1), cluster_size);
@@ -458,11 +468,12 @@ int slsf_sweep(slsf_batch* b, int32_t nsweeps, double* acceptance_sum) {
   clear_error();
   if (!b) return fail(SLSF_E_INVALID, "batch handle is null");
   if (!b->initialised) return fail(SLSF_E_STATE, "angles are not initialised");
-  if (int rc = ensure_trig_cache(b)) return rc;
+  if (int rc = before_per_phase(b)) return rc;
   if (nsweeps < 0) return fail(SLSF_E_INVALID, "nsweeps must be nonnegative");
   const int wmax = max_wolff_steps(b);
   launch_clear_counters(b->g, b->stream);
   launch_sweep_end(b->g, 0, 0, b->stream);
+  after_per_phase(b);
   for (int s = 0; s < nsweeps; ++s)
     if (int rc = hbm_sweep(b, wmax)) return rc;
   if (int rc = gather_acceptance(b, nsweeps, acceptance_sum)) return rc;
@@ -472,7 +483,7 @@ int slsf_sweep(slsf_batch* b, int32_t nsweeps, double* acceptance_sum) {
 int slsf_measure_observables(slsf_batch* b, slsf_observables* out) {
   clear_error();
   if (!b || !out) return fail(SLSF_E_INVALID, "null argument");
-  if (int rc = ensure_trig_cache(b)) return rc;
+  if (int rc = before_per_phase(b)) return rc;
   if (is_fp64(b)) launch_observables<double>(b->g, b->stream);
   else launch_observables<float>(b->g, b->stream);
   HIP_TRY(hipGetLastError());
@@ -711,7 +722,8 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     if (int rc = ensure_slab_buffers(b)) return rc;
     const bool slab = hbm_uses_slab(b->g, opt->hbm_form);
     if (slab) b->cs_stale = true;  // slab sweeps leave the trig cache behind
-    else if (int rc = ensure_trig_cache(b)) return rc;  // the per-phase kernels read it
+    else if (int rc = before_per_phase(b)) return rc;
+    after_per_phase(b);
     if (!resident_part) {
       launch_clear_counters(b->g, b->stream);
       launches += 2;
@@ -1092,6 +1104,7 @@ int driver_open(slsf_batch* b, int64_t nmeas, int rxy, int rz, DriverSession** o
 }
 
 int driver_sweeps(DriverSession* s, int nsweeps) {
+  after_per_phase(s->b);
   for (int k = 0; k < nsweeps; ++k)
     if (int rc = hbm_sweep(s->b, s->wmax)) return rc;
   return SLSF_OK;
@@ -1110,6 +1123,7 @@ int driver_measure(DriverSession* s, int64_t index) {
 int driver_correlations(DriverSession* s) {
   if (s->rxy + s->rz == 0) return SLSF_OK;
   slsf_batch* b = s->b;
+  if (int rc = ensure_trig_cache(b)) return rc;  // the row dot products read the cache
   const int64_t rows = (int64_t)b->reps[0].ly * b->reps[0].lz;
   if (is_fp64(b)) launch_corr<double>(b->g, 0, rows, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);
   else launch_corr<float>(b->g, 0, rows, s->rxy, s->rz, s->rows_xy, s->rows_z, b->stream);

@@ -123,7 +123,7 @@ struct Pool {
   std::vector<int32_t> active_idx;
   ResidentGroup group;            // active slots of a resident pool
   bool running = false;
-  bool cs_stale = false;          // slab chunks left the trig cache behind the angles
+  bool cs_stale = false;          // HBM chunks left the trig cache behind the angles
   std::vector<int64_t> done;      // each slot's sweep count after the last chunk
   double* corr_planes = nullptr;  // one correlation sample's plane sums
   std::vector<double> corr_host;
@@ -311,9 +311,11 @@ int64_t chunk_sweeps(slsf_server* s, const Pool& p) {
 // the host, added to the job's open bin.
 int sample_correlations(Pool& p, int slot, Job& j) {
   const int L = p.L, rxy = j.rxy, rz = j.rz;
-  if (p.cs_stale) {  // slab chunks left the trig cache behind the angles
-    launch_refresh<float>(p.g, p.stream);
-    p.cs_stale = false;
+  if (p.cs_stale) {  // HBM chunks left the trig cache behind the angles: this slot's pairs
+    GlobalState one = p.g;
+    one.reps = p.g.reps + slot;
+    one.nrep = 1;
+    launch_refresh<float>(one, p.stream);
   }
   launch_corr_planes<float>(p.g, slot, L, L, rxy, rz, p.corr_planes, p.stream);
   SRV_TRY(hipGetLastError());
@@ -379,10 +381,9 @@ int run_chunk(slsf_server* s, Pool& p) {
     for (int i : p.active_idx) wmax = std::max(wmax, p.host[i].wolff_steps);
     GlobalState g = p.g;  // launch only up to the highest active slot
     g.nrep = p.active_idx.back() + 1;
-    // the per-phase form reads the trig cache that slab chunks leave behind
-    const bool slab = hbm_uses_slab(g, 0);
-    if (!slab && p.cs_stale) launch_refresh<float>(p.g, p.stream);
-    p.cs_stale = slab;
+    // neither FP32 form keeps the trig cache: the correlation measurement
+    // refreshes it when it reads it
+    p.cs_stale = true;
     launch_sweep_end(g, 0, 0, p.stream);  // marks the slots whose window is open
     BinTarget bt;
     bt.accum = p.accum;
