@@ -239,10 +239,11 @@ def test_served_correlation_series_equal_batch_samples():
 
 def test_served_correlations_on_a_per_phase_pool():
     """An L=48 pool with twenty active jobs (540 16-deep tiles, above the
-    slab form's 512) runs the per-phase kernels, which keep the trig cache
-    current themselves: a correlation job among them (z and in-plane radii,
-    every measurement sampled) still gives the bin means of Batch's samples on
-    the same chain, bit for bit, and its neighbours keep their results."""
+    slab form's 256) runs the per-phase kernels; neither form keeps the trig
+    cache, so each correlation sample refreshes its slot's (cos, sin) pairs: a
+    correlation job among them (z and in-plane radii, every measurement
+    sampled) still gives the bin means of Batch's samples on the same chain,
+    bit for bit, and its neighbours keep their results."""
     corr = dict(job_args(48, 950, sweeps=12, therm=3, T=1.2, binsize=3),
                 correlation_interval=1, correlation_rmax_xy=2, correlation_rmax_z=3)
     others = [job_args(48, 960 + i, sweeps=12 + i % 3, therm=3, T=1.0 + 0.05 * i)
