@@ -234,18 +234,19 @@ inline void set_divisors(RepDev& d) {
 // address; a tally is the sum of its replica's slots.
 constexpr int TALLY_SLOTS = 64;
 
-// block partition of the fused observables: 2048 sites (8 per thread of a
-// 256-thread block) per block, blocks never straddle planes; a function of the
-// replica's own sizes only, so its FP64 sums do not depend on the batch
+// block partition of the fused observables: blocks never straddle planes; a
+// function of the replica's own sizes only, so its FP64 sums do not depend on
+// the batch.
 // A block of OBS_THREADS threads sums one chunk of a plane, obs_per_thread
-// sites per thread: 4 for planes of 1025 to 8192 sites and 8 below (more blocks
-// for the small single-replica lattices, which are latency-bound: 4 instead of
-// 8 gave +2-3% on one L=64 replica), 32 above (a quarter of the block
-// reductions where the sweep streams many large planes).  The partition
-// depends on the replica's extents only.
+// sites per thread: 4 for planes of 1025 to 8192 sites (more blocks for the
+// small single-replica lattices, which are latency-bound: 4 instead of 8 gave
+// +2-3% on one L=64 replica) and 8 otherwise (with block sums from wave
+// totals, 8 instead of 32 on the larger planes: four L=96 replicas 128 -> 122
+// us a sweep, four L=128 unchanged).  The partition depends on the replica's
+// extents only.
 constexpr int OBS_THREADS = 256;
 SLSF_HD int obs_per_thread(int lx, int ly) {
-  return lx * ly > 8192 ? 32 : (lx * ly > 1024 ? 4 : 8);
+  return lx * ly > 8192 ? 8 : (lx * ly > 1024 ? 4 : 8);
 }
 SLSF_HD int obs_chunks_per_plane(int lx, int ly) {
   const int chunk = OBS_THREADS * obs_per_thread(lx, ly);
