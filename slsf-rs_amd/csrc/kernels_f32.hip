@@ -44,7 +44,7 @@ void set_extents(GlobalState& g, const RepDev* reps, int n) {
     g.max_volume = std::max<int64_t>(g.max_volume, reps[r].volume);
     g.max_plane = std::max<int64_t>(g.max_plane, (int64_t)reps[r].lx * reps[r].ly);
     g.max_lz = std::max(g.max_lz, reps[r].lz);
-    g.obs_blocks = std::max(g.obs_blocks, obs_blocks_of(reps[r].lx, reps[r].ly, reps[r].lz));
+    g.obs_blocks = std::max(g.obs_blocks, measure_blocks_of(reps[r].lx, reps[r].ly, reps[r].lz));
     g.max_tiles = std::max(g.max_tiles, wolff_tiles_of<16>(reps[r].lx, reps[r].ly, reps[r].lz));
     g.max_tiles8 = std::max(g.max_tiles8, wolff_tiles_of<8>(reps[r].lx, reps[r].ly, reps[r].lz));
     g.max_tiles4 = std::max(g.max_tiles4, wolff_tiles_of<4>(reps[r].lx, reps[r].ly, reps[r].lz));
@@ -114,39 +114,58 @@ void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s
   k_sweep_end<<<(g.nrep + 63) / 64, 64, 0, s>>>(g.reps, g.nrep, metro, rounds);
 }
 
-// ---------------------------------------------------------------------------
-// per-measurement accumulation of the observable series into FP64 bins, on the
-// device: one workgroup per replica folds the block partials of k_observables
-// (fold_measurement's fixed order), forms the 13 series values and appends a
-// bin mean every `binsize` measurements.  A replica is measured iff it takes
-// part in this sweep and its counter (not yet advanced) is at least
-// measure_from.  With rounds >= 0 the same block then does the replica's
-// k_sweep_end(metro = 1, rounds) bookkeeping, which saves a launch on measured
-// sweeps.
-// ---------------------------------------------------------------------------
+// The end of a measured per-phase sweep (one workgroup per replica): the
+// measurement the colour-0 pass took of a pending replica is folded (its
+// metro_blocks_of partials, in block order), then the counters advance as in
+// k_sweep_end, and a replica measured in this sweep is marked pending for the
+// next colour-0 pass (src/model.rs:379-442, as the slab form's bookkeeping).
 __global__ void __launch_bounds__(FOLD_THREADS)
-k_accumulate(RepDev* reps, const double* __restrict__ partials, int obs_stride, double* accum,
-             double* bins, int32_t* bin_state, int32_t bin_cap, int rounds) {
+k_fold_pending(RepDev* reps, const double* __restrict__ partials, int stride, double* accum,
+               double* bins, int32_t* bin_state, int32_t bin_cap, int rounds) {
   const int r = blockIdx.x;
   RepDev& d = reps[r];
-  const bool measured = d.active && d.metro_ctr >= d.measure_from;
-  __syncthreads();  // every thread has read d before thread 0 advances it
-  if (measured)
-    fold_measurement(d, r, partials + (int64_t)r * obs_stride * 9,
-                     obs_blocks_of(d.lx, d.ly, d.lz), accum, bins, bin_state, bin_cap);
-  if (rounds >= 0 && threadIdx.x == 0) {  // as k_sweep_end(metro = 1, rounds), after every read of d
+  const bool fold = d.pending && accum;
+  __syncthreads();  // every thread has read the flag before thread 0 changes it
+  if (fold)
+    fold_measurement(d, r, partials + (int64_t)r * stride * 9, metro_blocks_of(d.lx, d.ly, d.lz),
+                     accum, bins, bin_state, bin_cap);
+  if (threadIdx.x == 0) {
+    const bool measured = d.active && d.metro_ctr >= d.measure_from;
     if (d.active) {
       d.metro_ctr += 1u;
       d.wolff_ctr += (uint32_t)(rounds < d.wolff_steps ? rounds : d.wolff_steps);
     }
     d.active = d.metro_ctr < d.stop_at ? 1u : 0u;
+    d.pending = (measured && accum) ? 1 : 0;
   }
 }
 
-void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t bin_cap, int rounds, hipStream_t s) {
-  k_accumulate<<<g.nrep, FOLD_THREADS, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins, bin_state,
-                                     bin_cap, rounds);
+// The closing measurement of a measured per-phase run: the replicas still
+// pending fold the partials k_observables took of them.
+__global__ void __launch_bounds__(FOLD_THREADS)
+k_close_observables(RepDev* reps, const double* __restrict__ partials, int stride, double* accum,
+                    double* bins, int32_t* bin_state, int32_t bin_cap) {
+  const int r = blockIdx.x;
+  RepDev& d = reps[r];
+  const bool pend = d.pending;
+  __syncthreads();  // every thread has read the flag before thread 0 clears it
+  if (!pend) return;
+  if (accum)
+    fold_measurement(d, r, partials + (int64_t)r * stride * 9, obs_blocks_of(d.lx, d.ly, d.lz),
+                     accum, bins, bin_state, bin_cap);
+  if (threadIdx.x == 0) d.pending = 0;
+}
+
+void launch_fold_pending(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
+                         int32_t bin_cap, int rounds, hipStream_t s) {
+  k_fold_pending<<<g.nrep, FOLD_THREADS, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins,
+                                                 bin_state, bin_cap, rounds);
+}
+
+void launch_close_observables(const GlobalState& g, double* accum, double* bins,
+                              int32_t* bin_state, int32_t bin_cap, hipStream_t s) {
+  k_close_observables<<<g.nrep, FOLD_THREADS, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum,
+                                                      bins, bin_state, bin_cap);
 }
 
 // ---------------------------------------------------------------------------

@@ -30,7 +30,7 @@ struct GlobalState {
   long long* csize = nullptr;         // [nrep * TALLY_SLOTS] cluster sites, summed until cleared
   unsigned long long* acc = nullptr;  // [nrep * TALLY_SLOTS] accepted moves, summed until cleared
   double* partials = nullptr;         // [nrep * obs_blocks * 9]
-  int obs_blocks = 0;                 // max over replicas of obs_blocks_of
+  int obs_blocks = 0;                 // max over replicas of measure_blocks_of
   // slab sweeps (enqueue_hbm_sweeps): the second angle buffer, R[total_sites],
   // and the measurement partials of the quarter tiles, [nrep * max_tiles4 * 9]
   void* theta2 = nullptr;
@@ -44,18 +44,21 @@ template <typename R> void launch_init(const GlobalState& g, int cold, hipStream
 template <typename R> void launch_refresh(const GlobalState& g, hipStream_t s);
 // A sweep of the HBM path: launch_sweep_end(g, 0, 0) marks the replicas whose
 // window is open, then per sweep launch_metro_phases, launch_wolff for rounds
-// 0..wmax-1, then either launch_observables + launch_accumulate(.., wmax, ..)
-// or launch_sweep_end(g, 1, wmax), which advance the counters and mark the
-// next sweep.
+// 0..wmax-1, then launch_sweep_end(g, 1, wmax) (or, in a measured run,
+// launch_fold_pending), which advance the counters and mark the next sweep
+// (enqueue_hbm_sweeps composes the sequences).
+// With measure (a measured run of per-phase sweeps) the colour-0 phase also
+// measures the replicas whose last sweep is pending, into g.partials.
 template <typename R> void launch_metro_phases(const GlobalState& g, uint8_t* decisions,
-                                               hipStream_t s);
+                                               hipStream_t s, bool measure = false);
 // cluster round `round` (replicas with wolff_steps <= round sit it out): seed,
 // bond activations, union-find, reflection; the membership bitmap stays valid
 // until the next round.
 template <typename R> void launch_wolff(const GlobalState& g, int round, hipStream_t s);
 void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s);
 // windowed = 1: only replicas measured in this sweep (active, counter at least
-// measure_from) compute their partials; 0: every replica (a direct measurement)
+// measure_from) compute their partials; 2: only replicas whose last sweep is
+// pending; 0: every replica (a direct measurement)
 template <typename R> void launch_observables(const GlobalState& g, hipStream_t s,
                                               int windowed = 0);
 // rows = ly * lz of that replica (sizes the grid)
@@ -76,10 +79,16 @@ void launch_window(const GlobalState& g, uint32_t therm, uint32_t total, int32_t
 // out[r] = sum of replica r's TALLY_SLOTS slots
 void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
                         unsigned long long* out, hipStream_t s);
-// folds a measured sweep into the bins; with rounds >= 0 also advances the
-// counters as launch_sweep_end(g, 1, rounds) would
-void launch_accumulate(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                       int32_t bin_cap, int rounds, hipStream_t s);
+// the end of a measured per-phase sweep: folds the colour-0 pass's partials of
+// the replicas that were pending, advances the counters as
+// launch_sweep_end(g, 1, rounds), and marks the replicas measured in this
+// sweep as pending
+void launch_fold_pending(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
+                         int32_t bin_cap, int rounds, hipStream_t s);
+// the end of a measured per-phase run: folds the partials of
+// launch_observables(g, s, 2) of the pending replicas and clears the flag
+void launch_close_observables(const GlobalState& g, double* accum, double* bins,
+                              int32_t* bin_state, int32_t bin_cap, hipStream_t s);
 
 // The FP64 replay of one small replica inside one workgroup
 // (k_replay_resident): a chunk of `nsweeps` sweeps of the plain driver, with

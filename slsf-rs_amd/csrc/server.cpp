@@ -57,7 +57,7 @@ int64_t slot_bytes(int L) {
   int64_t b = vp * 12 + 2 * L * 4 + (int64_t)sizeof(RepDev) + 13 * 8 + 2 * 4 +
               13 * 8 * (int64_t)kBinCap + 2 * 8 * TALLY_SLOTS + 4;
   if (!resident_supports(L))
-    b += vp * 5 + vp / 8 + (int64_t)obs_blocks_of(L, L, L) * 9 * 8 + (int64_t)sizeof(WolffDev) +
+    b += vp * 5 + vp / 8 + (int64_t)measure_blocks_of(L, L, L) * 9 * 8 + (int64_t)sizeof(WolffDev) +
          vp * 4 + (int64_t)((L + 15) / 16) * ((L + 15) / 16) * ((L + 3) / 4) * 9 * 8;
   return b;
 }

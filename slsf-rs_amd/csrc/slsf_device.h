@@ -254,6 +254,21 @@ SLSF_HD int obs_chunks_per_plane(int lx, int ly) {
 }
 SLSF_HD int obs_blocks_of(int lx, int ly, int lz) { return lz * obs_chunks_per_plane(lx, ly); }
 
+// k_metro: METRO_SITES sites of one colour a thread, 256 threads a block, the
+// blocks of a plane's colour half numbered from 0; the per-phase sweeps'
+// measurement rides on the colour-0 pass, one partial per such block (its
+// count, like the partition, depends on the replica's extents only)
+constexpr int METRO_SITES = 2;
+SLSF_HD int metro_blocks_per_plane(int lx, int ly) {
+  return ((lx / 2) * ly + 256 * METRO_SITES - 1) / (256 * METRO_SITES);
+}
+SLSF_HD int metro_blocks_of(int lx, int ly, int lz) { return lz * metro_blocks_per_plane(lx, ly); }
+// partials a replica needs for either measurement of the per-phase sweeps
+SLSF_HD int measure_blocks_of(int lx, int ly, int lz) {
+  const int a = obs_blocks_of(lx, ly, lz), b = metro_blocks_of(lx, ly, lz);
+  return a > b ? a : b;
+}
+
 struct WolffDev {  // per replica, per cluster step
   double phi, sphi, cphi;
   int64_t seed_site;
