@@ -90,18 +90,16 @@ void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
   k_sum_tallies<<<(g.nrep + 63) / 64, 64, 0, s>>>(slots, g.nrep, out);
 }
 
-// The slab form wins while a launch has at most one 16-deep tile per compute
-// unit, or is a single replica of up to two (measured per sweep, FP32, slab
-// against per-phase: one L=96 replica 60 against 70 us, four L=64 60 against
-// 60, eight L=48 56 against 57, one L=128 92 against 95; six L=64 85 against
-// 74, twelve L=48 83 against 72, two L=96 88 against 87); beyond that the
-// per-phase kernels' small independent blocks fill the device better (four
-// L=128 replicas 276 against 217 us).
+// The slab form wins while a launch has at most one 16-deep tile per two
+// compute units (measured per sweep, FP32, slab against per-phase: one L=64
+// replica 34.3 against 41.8 us, four L=48 41.3 against 44.5, two L=64 42.2
+// against 45.1); beyond that the per-phase kernels' small independent blocks
+// fill the device better (eight L=48 55.9 against 52.2, one L=96 59.2 against
+// 56.6, four L=64 57.7 against 54.5, one L=128 93.2 against 83.3).
 bool hbm_uses_slab(const GlobalState& g, int form) {
   if (form == 1) return true;
   if (form == 2) return false;
-  const int64_t tiles = (int64_t)g.max_tiles * g.nrep;
-  return tiles <= WOLFF_SHALLOW_BELOW || (g.nrep == 1 && tiles <= 2 * WOLFF_SHALLOW_BELOW);
+  return (int64_t)g.max_tiles * g.nrep <= WOLFF_SHALLOW_BELOW / 2;
 }
 
 void launch_clear_counters(const GlobalState& g, hipStream_t s) {

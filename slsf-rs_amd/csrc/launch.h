@@ -162,8 +162,7 @@ template <typename R> int64_t enqueue_hbm_sweeps(const GlobalState& g, int64_t n
                                                  const BinTarget& bt, hipStream_t s,
                                                  int form = 0);
 // whether enqueue_hbm_sweeps takes the slab form: forced (form 1 / 2) or, for
-// form 0, when the launch has at most one 16-deep tile per compute unit, or is
-// a single replica of at most two
+// form 0, when the launch has at most one 16-deep tile per two compute units
 bool hbm_uses_slab(const GlobalState& g, int form);
 
 }  // namespace slsf
