@@ -437,19 +437,18 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
   };
 
   // folds the partial sums of one measurement into the replica's FP64 bins:
-  // wave sums (shuffles), then wave 0 finishes with one lane per quantity --
-  // lane k < 9 adds quantity k over the waves in wave order, lane k < 13
-  // accumulates series k -- against the accumulators held in LDS for the
-  // launch, so the other waves never wait on a global memory round trip
+  // wave sums (DPP lane moves and row swaps, no LDS round trips), then wave 0
+  // finishes with one lane per quantity -- lane k < 8 adds quantity k over
+  // the waves in wave order, lane k < 13 accumulates series k -- against the
+  // accumulators held in LDS for the launch, so the other waves never wait on
+  // a global memory round trip
   auto measure_finish = [&]() __attribute__((always_inline)) {
     // quantities: mx, my, cos_x..z, sin_x..z (the energy is -(cos_x + cos_y +
     // cos_z), formed from their FP64 totals)
     double v[8] = {(double)part.mx,    (double)part.my,    (double)part.ec[0], (double)part.ec[1],
                    (double)part.ec[2], (double)part.es[0], (double)part.es[1], (double)part.es[2]};
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    for (int k = 0; k < 8; ++k) v[k] = wave_total_f64(v[k]);
     if ((t & 63) == 0)
 #pragma unroll
       for (int k = 0; k < 8; ++k) red[(t >> 6) * 8 + k] = v[k];

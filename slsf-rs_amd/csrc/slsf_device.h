@@ -278,4 +278,52 @@ struct WolffDev {  // per replica, per cluster step
   int32_t pad;
 };
 
+
+// ---- wave sums of doubles in a fixed order ---------------------------------
+// A DPP lane move of a double (both 32-bit halves) under control CTRL.
+template <int CTRL>
+SLSF_D double dpp_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// v + v[l ^ 16] and v + v[l ^ 32] in every lane l, from gfx950's row swaps:
+// v_permlane16_swap(a, b) leaves rows (a0, b0, a2, b2) in its first result
+// and (a1, b1, a3, b3) in its second (measured), v_permlane32_swap the two
+// halves likewise, so with a = b = v the two results hold a lane's own row and
+// its partner row, and their sum is the pair sum.  VALU moves instead of two
+// LDS round trips (ds_bpermute) per 32-bit half.
+SLSF_D double swap_sum16_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+  const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return __builtin_bit_cast(double, ((uint64_t)ph[0] << 32) | pl[0]) +
+         __builtin_bit_cast(double, ((uint64_t)ph[1] << 32) | pl[1]);
+}
+SLSF_D double swap_sum32_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+  const auto pl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto ph = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __builtin_bit_cast(double, ((uint64_t)ph[0] << 32) | pl[0]) +
+         __builtin_bit_cast(double, ((uint64_t)ph[1] << 32) | pl[1]);
+}
+// Sum over the 64 lanes of a wave, every lane receiving the total: pairs,
+// quads, half rows and rows of 16 through DPP lane moves (each partial formed
+// from the same two operands on both lanes that hold it), then the row sums
+// r0 + r1 and (r0 + r1) + (r2 + r3) through row swaps (IEEE addition commutes,
+// so every lane gets the one value).  The order of wave_sum_f64, bit for bit,
+// without its scalar reads: no scalar registers, for kernels whose occupancy
+// their SGPR count sets.
+SLSF_D double wave_total_f64(double v) {
+  v = v + dpp_f64<0xB1>(v);
+  v = v + dpp_f64<0x4E>(v);
+  v = v + dpp_f64<0x141>(v);
+  v = v + dpp_f64<0x140>(v);
+  return swap_sum32_f64(swap_sum16_f64(v));
+}
+
 }  // namespace slsf
