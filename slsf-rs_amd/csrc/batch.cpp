@@ -708,6 +708,7 @@ int slsf_run(slsf_batch* b, const slsf_run_options* opt, slsf_run_stats* stats) 
     a.acc_out = b->g.acc;
     a.csize_out = b->g.csize;
     ResidentRunResult res;  // zeroes the tallies of every replica first
+    b->cs_stale = true;     // the resident kernel writes angles only
     if (int rc = resident_run(b->resident, b->g, a, b->stream, &res)) return rc;
     launches += res.launches;
     kernel_ms += res.kernel_ms;

@@ -59,7 +59,7 @@ struct slsf_batch {
   std::vector<void*> allocations;
   ResidentPlan resident;         // FP32 register/LDS-resident groups
   bool initialised = false;
-  bool cs_stale = false;         // slab sweeps left the (cos, sin) cache behind theta
+  bool cs_stale = false;         // FP32 sweeps left the (cos, sin) cache behind theta
 
   ~slsf_batch() {
     resident_release(resident);

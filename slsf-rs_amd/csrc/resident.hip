@@ -66,7 +66,6 @@ struct ResArgs {
   RepDev* reps;
   const int32_t* index;  // batch replica index per workgroup
   float* theta;
-  float2* cs;
   const float* jxy;
   const float* jz;
   double* accum;
@@ -770,7 +769,9 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
     measure();
   }
 
-  // write back the angles (turns) and their trig cache, counters, tallies
+  // write back the angles (turns), counters, tallies.  The (cos, sin) cache in
+  // HBM is left behind the angles: the library refreshes it (with the same
+  // v_sin/v_cos) where a caller reads it, which a launch of sweeps does not
   if (own) {
     const int zpar = (x + y) & 1;
 #pragma unroll
@@ -781,9 +782,7 @@ __global__ void __launch_bounds__((RShape<L, CPT>::NT)) k_resident(ResArgs a) {
         for (int k = 0; k < H; ++k) {
           const int z = 2 * k + (c ^ zpar);
           const int64_t g = base + x + L * (y + j * S::YS + L * z);
-          const float v = opq(th[j][c * H + k]);
-          a.theta[g] = v;
-          a.cs[g] = make_float2(cos_t(v), sin_t(v));
+          a.theta[g] = th[j][c * H + k];
         }
   }
   if (has_bins && t < 13) a.accum[(int64_t)rep * 13 + t] = sm.acc[t];
@@ -882,7 +881,6 @@ static ResArgs make_args(const ResidentGroup& grp, const GlobalState& g,
   a.reps = g.reps;
   a.index = grp.d_index;
   a.theta = (float*)g.theta;
-  a.cs = (float2*)g.cs;
   a.jxy = (const float*)g.jxy;
   a.jz = (const float*)g.jz;
   a.accum = ra.accum;

@@ -123,7 +123,7 @@ struct Pool {
   std::vector<int32_t> active_idx;
   ResidentGroup group;            // active slots of a resident pool
   bool running = false;
-  bool cs_stale = false;          // HBM chunks left the trig cache behind the angles
+  bool cs_stale = false;          // chunks left the trig cache behind the angles
   std::vector<int64_t> done;      // each slot's sweep count after the last chunk
   double* corr_planes = nullptr;  // one correlation sample's plane sums
   std::vector<double> corr_host;
@@ -311,7 +311,7 @@ int64_t chunk_sweeps(slsf_server* s, const Pool& p) {
 // the host, added to the job's open bin.
 int sample_correlations(Pool& p, int slot, Job& j) {
   const int L = p.L, rxy = j.rxy, rz = j.rz;
-  if (p.cs_stale) {  // HBM chunks left the trig cache behind the angles: this slot's pairs
+  if (p.cs_stale) {  // chunks left the trig cache behind the angles: this slot's pairs
     GlobalState one = p.g;
     one.reps = p.g.reps + slot;
     one.nrep = 1;
@@ -376,6 +376,7 @@ int run_chunk(slsf_server* s, Pool& p) {
     ra.acc_out = p.g.acc;
     ra.csize_out = p.g.csize;
     resident_enqueue(p.group, p.g, ra, p.stream);
+    p.cs_stale = true;  // the resident kernel writes angles only
   } else {
     int wmax = 0;
     for (int i : p.active_idx) wmax = std::max(wmax, p.host[i].wolff_steps);
