@@ -112,7 +112,8 @@ void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s
   k_sweep_end<<<(g.nrep + 63) / 64, 64, 0, s>>>(g.reps, g.nrep, metro, rounds);
 }
 
-// The end of a measured per-phase sweep (one workgroup per replica): the
+// The end of a measured per-phase sweep without cluster rounds (one workgroup
+// per replica; with rounds, the last round's reflection kernel does it): the
 // measurement the colour-0 pass took of a pending replica is folded (its
 // metro_blocks_of partials, in block order), then the counters advance as in
 // k_sweep_end, and a replica measured in this sweep is marked pending for the
@@ -122,20 +123,8 @@ k_fold_pending(RepDev* reps, const double* __restrict__ partials, int stride, do
                double* bins, int32_t* bin_state, int32_t bin_cap, int rounds) {
   const int r = blockIdx.x;
   RepDev& d = reps[r];
-  const bool fold = d.pending && accum;
-  __syncthreads();  // every thread has read the flag before thread 0 changes it
-  if (fold)
-    fold_measurement(d, r, partials + (int64_t)r * stride * 9, metro_blocks_of(d.lx, d.ly, d.lz),
-                     accum, bins, bin_state, bin_cap);
-  if (threadIdx.x == 0) {
-    const bool measured = d.active && d.metro_ctr >= d.measure_from;
-    if (d.active) {
-      d.metro_ctr += 1u;
-      d.wolff_ctr += (uint32_t)(rounds < d.wolff_steps ? rounds : d.wolff_steps);
-    }
-    d.active = d.metro_ctr < d.stop_at ? 1u : 0u;
-    d.pending = (measured && accum) ? 1 : 0;
-  }
+  sweep_bookkeeping(d, r, partials + (int64_t)r * stride * 9, metro_blocks_of(d.lx, d.ly, d.lz),
+                    accum, bins, bin_state, bin_cap, rounds);
 }
 
 // The closing measurement of a measured per-phase run: the replicas still

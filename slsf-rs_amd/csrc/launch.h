@@ -51,10 +51,24 @@ template <typename R> void launch_refresh(const GlobalState& g, hipStream_t s);
 // measures the replicas whose last sweep is pending, into g.partials.
 template <typename R> void launch_metro_phases(const GlobalState& g, uint8_t* decisions,
                                                hipStream_t s, bool measure = false);
+// The bookkeeping at the end of a per-phase sweep: with accum set, the
+// colour-0 pass's partials of the replicas that were pending are folded into
+// their bins and the replicas measured in this sweep marked pending
+// (launch_fold_pending); either way the counters advance as
+// launch_sweep_end(g, 1, rounds) does.
+struct SweepEnd {
+  double* accum = nullptr;
+  double* bins = nullptr;
+  int32_t* bin_state = nullptr;
+  int32_t bin_cap = 0;
+  int rounds = 0;
+};
 // cluster round `round` (replicas with wolff_steps <= round sit it out): seed,
 // bond activations, union-find, reflection; the membership bitmap stays valid
-// until the next round.
-template <typename R> void launch_wolff(const GlobalState& g, int round, hipStream_t s);
+// until the next round.  With `end` (the sweep's last round) the reflection
+// kernel also does the sweep's bookkeeping, one workgroup per replica.
+template <typename R> void launch_wolff(const GlobalState& g, int round, hipStream_t s,
+                                        const SweepEnd* end = nullptr);
 void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s);
 // windowed = 1: only replicas measured in this sweep (active, counter at least
 // measure_from) compute their partials; 2: only replicas whose last sweep is
