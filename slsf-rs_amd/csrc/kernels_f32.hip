@@ -120,11 +120,12 @@ void launch_sweep_end(const GlobalState& g, int metro, int rounds, hipStream_t s
 // next colour-0 pass (src/model.rs:379-442, as the slab form's bookkeeping).
 __global__ void __launch_bounds__(FOLD_THREADS)
 k_fold_pending(RepDev* reps, const double* __restrict__ partials, int stride, double* accum,
-               double* bins, int32_t* bin_state, int32_t bin_cap, int rounds) {
+               double* bins, int32_t* bin_state, int32_t bin_cap, int rounds, int ms) {
   const int r = blockIdx.x;
   RepDev& d = reps[r];
-  sweep_bookkeeping(d, r, partials + (int64_t)r * stride * 9, metro_blocks_of(d.lx, d.ly, d.lz),
-                    accum, bins, bin_state, bin_cap, rounds);
+  sweep_bookkeeping(d, r, partials + (int64_t)r * stride * 9,
+                    metro_blocks_of(d.lx, d.ly, d.lz, ms), accum, bins, bin_state, bin_cap,
+                    rounds);
 }
 
 // The closing measurement of a measured per-phase run: the replicas still
@@ -144,9 +145,9 @@ k_close_observables(RepDev* reps, const double* __restrict__ partials, int strid
 }
 
 void launch_fold_pending(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                         int32_t bin_cap, int rounds, hipStream_t s) {
+                         int32_t bin_cap, int rounds, int measure_sites, hipStream_t s) {
   k_fold_pending<<<g.nrep, FOLD_THREADS, 0, s>>>(g.reps, g.partials, g.obs_blocks, accum, bins,
-                                                 bin_state, bin_cap, rounds);
+                                                 bin_state, bin_cap, rounds, measure_sites);
 }
 
 void launch_close_observables(const GlobalState& g, double* accum, double* bins,

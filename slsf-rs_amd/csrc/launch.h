@@ -97,8 +97,10 @@ void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
 // the replicas that were pending, advances the counters as
 // launch_sweep_end(g, 1, rounds), and marks the replicas measured in this
 // sweep as pending
+// (measure_sites: sites a thread of the measuring colour-0 pass, which sets
+// the number of partials)
 void launch_fold_pending(const GlobalState& g, double* accum, double* bins, int32_t* bin_state,
-                         int32_t bin_cap, int rounds, hipStream_t s);
+                         int32_t bin_cap, int rounds, int measure_sites, hipStream_t s);
 // the end of a measured per-phase run: folds the partials of
 // launch_observables(g, s, 2) of the pending replicas and clears the flag
 void launch_close_observables(const GlobalState& g, double* accum, double* bins,

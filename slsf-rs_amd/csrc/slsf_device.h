@@ -255,17 +255,25 @@ SLSF_HD int obs_chunks_per_plane(int lx, int ly) {
 SLSF_HD int obs_blocks_of(int lx, int ly, int lz) { return lz * obs_chunks_per_plane(lx, ly); }
 
 // k_metro: METRO_SITES sites of one colour a thread, 256 threads a block, the
-// blocks of a plane's colour half numbered from 0; the per-phase sweeps'
-// measurement rides on the colour-0 pass, one partial per such block (its
-// count, like the partition, depends on the replica's extents only)
+// blocks of a plane's colour half numbered from 0.  The per-phase sweeps'
+// measurement rides on the colour-0 pass, whose FP32 threads take
+// 2 METRO_SITES sites each (in groups of METRO_SITES), so that a block
+// reduction of the measurement serves twice the sites (FP64 threads, whose
+// registers are full already, keep METRO_SITES); one partial per block of
+// that pass (its count, like the partition, depends on the replica's extents
+// and the precision only)
 constexpr int METRO_SITES = 2;
-SLSF_HD int metro_blocks_per_plane(int lx, int ly) {
-  return ((lx / 2) * ly + 256 * METRO_SITES - 1) / (256 * METRO_SITES);
+template <typename R>
+constexpr int metro_measure_sites() { return sizeof(R) == 4 ? 2 * METRO_SITES : METRO_SITES; }
+SLSF_HD int metro_blocks_per_plane(int lx, int ly, int ms) {
+  return ((lx / 2) * ly + 256 * ms - 1) / (256 * ms);
 }
-SLSF_HD int metro_blocks_of(int lx, int ly, int lz) { return lz * metro_blocks_per_plane(lx, ly); }
+SLSF_HD int metro_blocks_of(int lx, int ly, int lz, int ms) {
+  return lz * metro_blocks_per_plane(lx, ly, ms);
+}
 // partials a replica needs for either measurement of the per-phase sweeps
 SLSF_HD int measure_blocks_of(int lx, int ly, int lz) {
-  const int a = obs_blocks_of(lx, ly, lz), b = metro_blocks_of(lx, ly, lz);
+  const int a = obs_blocks_of(lx, ly, lz), b = metro_blocks_of(lx, ly, lz, METRO_SITES);
   return a > b ? a : b;
 }
 
