@@ -195,6 +195,29 @@ def test_slab_and_per_phase_forms_give_the_same_chains_and_bins(precision):
         assert np.array_equal(ctr[0], ref[3][0]) and np.array_equal(ctr[1], ref[3][1])
 
 
+def test_forms_agree_on_metropolis_only_batches():
+    """A batch whose replicas take no cluster rounds ends its per-phase sweeps
+    with the fold kernel of its own instead of the reflection kernel's
+    bookkeeping: chains and counters equal the slab form's, bins to 1e-12."""
+    specs = [(48, 48, 48, 1.3, 5, 0), (10, 30, 12, 1.6, 6, 0)]
+    reps = [_pair(lx, ly, lz, T, seed, ws)[0] for lx, ly, lz, T, seed, ws in specs]
+    out = []
+    for form in (1, 2):
+        b = slsf.Batch(reps, slsf.FP32)
+        b.init_angles()
+        b.run(2, 6, 2, hbm_form=form)
+        out.append(([b.get_theta(i) for i in range(len(reps))],
+                    [b.all_bins(i) for i in range(len(reps))], b.counters()))
+        b.close()
+    for i in range(len(reps)):
+        assert np.array_equal(out[0][0][i], out[1][0][i]), i
+        assert out[0][1][i].shape[1] == 3
+        assert np.allclose(out[0][1][i], out[1][1][i], rtol=1e-12, atol=1e-12), i
+    for k in range(2):
+        assert np.array_equal(out[0][2][k], out[1][2][k])
+    assert list(out[1][2][0]) == [8, 8] and list(out[1][2][1]) == [0, 0]
+
+
 def test_hbm_form_is_checked():
     b = slsf.Batch([_pair(48, 48, 48, 1.3, 1, 1)[0]], slsf.FP32)
     b.init_angles()
