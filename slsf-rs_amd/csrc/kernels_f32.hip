@@ -92,10 +92,10 @@ void launch_sum_tallies(const GlobalState& g, const unsigned long long* slots,
 
 // The slab form wins while a launch has at most one 16-deep tile per two
 // compute units (measured per sweep, FP32, slab against per-phase: one L=64
-// replica 34.3 against 41.8 us, four L=48 41.3 against 44.5, two L=64 42.2
-// against 45.1); beyond that the per-phase kernels' small independent blocks
-// fill the device better (eight L=48 55.9 against 52.2, one L=96 59.2 against
-// 56.6, four L=64 57.7 against 54.5, one L=128 93.2 against 83.3).
+// replica 34.7 against 39.7 us, four L=48 41.5 against 42.6, two L=64 42.6
+// against 43.2); beyond that the per-phase kernels' small independent blocks
+// fill the device better (eight L=48 55.7 against 50.7, one L=96 59.7 against
+// 54.2, four L=64 59.0 against 53.2, one L=128 93.2 against 83.3).
 bool hbm_uses_slab(const GlobalState& g, int form) {
   if (form == 1) return true;
   if (form == 2) return false;
