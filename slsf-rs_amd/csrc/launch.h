@@ -167,9 +167,11 @@ struct BinTarget {
 //    Three launches a sweep, which suits launches too small to fill the device
 //    (their time is the latency of the dependent launches).  Leaves the (cos,
 //    sin) cache stale (launch_refresh).
-//  * per-phase sweeps: two Metropolis colours, three cluster launches per
-//    round, the measurement and the counters (seven launches a sweep), whose
-//    small independent blocks fill the device better when the launch is large.
+//  * per-phase sweeps: two Metropolis colours (the first also measures the
+//    state the sweep starts from) and three cluster launches per round, the
+//    last reflection also folding the measurement and advancing the counters
+//    (five launches a sweep), whose small independent blocks fill the device
+//    better when the launch is large.
 // form 1 and 2 force the slab and the per-phase form; 0 chooses by the launch
 // size (hbm_uses_slab).  Both need g.theta2 and g.slab_partials.  Every sweep
 // issues the same launches (the replicas' windows decide who takes part), so the
