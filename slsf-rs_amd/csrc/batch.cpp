@@ -963,7 +963,7 @@ int slsf_checkpoint_save(slsf_batch* b, void* buffer, int64_t bytes) {
     std::memcpy(p, &c, sizeof c);
     p += sizeof c;
   }
-  // angles and their trig cache exactly as the kernels hold them
+  // angles and their trig cache (refreshed above) exactly as the kernels hold them
   const size_t tb = (size_t)h.total_sites * real_bytes(b);
   HIP_TRY(hipMemcpy(p, b->g.theta, tb, hipMemcpyDeviceToHost));
   p += tb;

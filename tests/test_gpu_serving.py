@@ -239,7 +239,7 @@ def test_served_correlation_series_equal_batch_samples():
 
 def test_served_correlations_on_a_per_phase_pool():
     """An L=48 pool with twenty active jobs (540 16-deep tiles, above the
-    slab form's 128) runs the per-phase kernels; neither form keeps the trig
+    slab form's 128) runs the per-phase kernels; no FP32 kernel keeps the trig
     cache, so each correlation sample refreshes its slot's (cos, sin) pairs: a
     correlation job among them (z and in-plane radii, every measurement
     sampled) still gives the bin means of Batch's samples on the same chain,
