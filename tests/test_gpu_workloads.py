@@ -111,7 +111,8 @@ def test_fp32_matches_fp64_cpu_reference_mode_at_l32():
     """Realizations 0..15 of the L=32 workload at T = 1.30: the GPU's FP32
     production chains against the oracle's FP64 reference-mode chains on the
     host (independent streams), 500 + 2500 sweeps each, bins of 100.  Tolerance
-    (north star): the disorder-averaged observables within 2 combined sigma.
+    (the FP32 production accuracy bar): the disorder-averaged observables within
+    2 combined sigma.
     Five observables held to 2 sigma each would fail about one seed set in four
     by chance alone (1 - 0.95^5), so one of the five may lie between 2 and 3
     sigma (a chance of about 3% to fail when they agree)."""
